@@ -1,0 +1,245 @@
+/*
+ * lofloat.h -- C-ABI of the MI355X (gfx950) LoFloat hot-path library
+ *              (liblofloat_hip.so).
+ *
+ * Plain pointers, sizes and PODs only: no torch, no C++ types, no exceptions.
+ * Every entry point is stream-ordered, performs no host synchronisation and no
+ * device allocation, and may be captured into a HIP graph.  The one exception
+ * is lof_grouped_create(), which allocates the plan's device descriptors and
+ * copies them synchronously; create plans before capture.
+ *
+ * Device pointers are HIP device memory on the current device.  `stream` is a
+ * hipStream_t (NULL = the legacy default stream).  Errors come back as a
+ * lof_status; lof_last_error() gives the message for the calling thread.
+ *
+ * Numerics: results are bit-identical to the reference library's scalar
+ * semantics (SudhanvaKulkarni123/LoFloat, src/lo_float.h virtual_round),
+ * quirks included (saturating overflow returns the unrounded value, extended
+ * overflow returns +inf, NaN becomes +qNaN, ...).  Stochastic modes consume
+ * one uint32 random word per element: from `rand_bits[i]` when given, else
+ * the counter-based word lof_rand_u32(seed, i).
+ */
+#ifndef LOFLOAT_H
+#define LOFLOAT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LOF_API __attribute__((visibility("default")))
+
+typedef void* lof_stream_t; /* hipStream_t */
+
+typedef enum lof_status {
+    LOF_OK = 0,
+    LOF_ERR_INVALID_ARGUMENT = 1,
+    LOF_ERR_UNSUPPORTED = 2,
+    LOF_ERR_HIP = 3,
+    LOF_ERR_ALLOC = 4
+} lof_status;
+
+/* Rounding_Mode, in the reference's X-macro order (fp_tools.hpp:13-28). */
+enum {
+    LOF_RM_ROUND_TO_NEAREST_EVEN = 0,
+    LOF_RM_ROUND_TOWARDS_ZERO = 1,
+    LOF_RM_ROUND_AWAY_FROM_ZERO = 2,
+    LOF_RM_STOCHASTIC_A = 3,
+    LOF_RM_ROUND_TO_NEAREST_ODD = 4,
+    LOF_RM_ROUND_DOWN = 5,
+    LOF_RM_ROUND_UP = 6,
+    LOF_RM_ROUND_TIES_TO_AWAY = 7,
+    LOF_RM_STOCHASTIC_B = 8,
+    LOF_RM_STOCHASTIC_C = 9,
+    LOF_RM_TRUE_STOCHASTIC = 10,
+    LOF_RM_PROBABILISTIC = 11,
+    LOF_RM_ROUND_TO_ODD = 12,
+    LOF_RM_ROUND_TIES_TOWARDS_ZERO = 13,
+    LOF_RM_STOCHASTIC_D = 14
+};
+
+/* Inf_Behaviors / NaN_Behaviors / Signedness (fp_tools.hpp:30-50). */
+enum { LOF_INF_EXTENDED = 0, LOF_INF_SATURATING = 1 };
+enum { LOF_NAN_3109 = 0, LOF_NAN_754 = 1 };
+enum { LOF_SIGNED = 0, LOF_UNSIGNED = 1 };
+
+/* FloatingPointParams without the checker functors, which virtual_round never
+ * reads (fp_tools.hpp:187-252; LoPy_bind.cpp:178-185). */
+typedef struct lof_format {
+    int32_t bitwidth;      /* total bits incl. sign */
+    int32_t mantissa_bits; /* 0..23 */
+    int32_t bias;
+    int32_t inf_behavior;  /* LOF_INF_* */
+    int32_t nan_behavior;  /* LOF_NAN_* (not read by rounding) */
+    int32_t signedness;    /* LOF_SIGNED / LOF_UNSIGNED */
+} lof_format;
+
+/* ProjSpec (fp_tools.hpp:78-88). */
+typedef struct lof_round_spec {
+    int32_t mode;       /* LOF_RM_* */
+    int32_t saturation; /* Saturation_Mode; not read by virtual_round */
+    int32_t stoch_len;  /* random bits for SR-A/B/C/D; 0 <= stoch_len <= 23 - mantissa_bits */
+} lof_round_spec;
+
+LOF_API const char* lof_last_error(void);
+LOF_API int32_t lof_version(void);
+
+/* Random word `index` of stream `seed` (SplitMix64 output over a Weyl sequence):
+ * what the kernels use when no rand_bits array is passed.  Host-callable. */
+LOF_API uint32_t lof_rand_u32(uint64_t seed, uint64_t index);
+
+/* Fill bits[0..n) with lof_rand_u32(seed, offset + i) on the device. */
+LOF_API lof_status lof_fill_random_u32(uint32_t* bits, int64_t n, uint64_t seed, uint64_t offset,
+                                       lof_stream_t stream);
+
+/* out[i] = virtual_round(v_i, *fmt, spec), v_i = (scale == 1) ? in[i] : scale * in[i].
+ * Replaces lo_float::round_fp_params (Lof_kernel.cu:49-60; Lof_kernels.h via
+ * LoPy_bind.cpp:17-18) and the CUDA branch of virtual_round_params
+ * (LoPy_bind.cpp:122-139).  in == out is allowed.  rand_bits: NULL or n words. */
+LOF_API lof_status lof_virtual_round_f32(const float* in, float* out, int64_t n,
+                                         const lof_format* fmt, lof_round_spec spec, float scale,
+                                         const uint32_t* rand_bits, uint64_t seed,
+                                         lof_stream_t stream);
+
+/* out[i] = virtual_round(v_i, mant_bits, spec) (mantissa-only overload,
+ * lo_float.h:2716-2756).  Replaces lo_float::round_mantissa (Lof_kernel.cu:37-48;
+ * LoPy_bind.cpp:14-15, 80-120).  0 <= mant_bits <= 23. */
+LOF_API lof_status lof_virtual_round_mant_f32(const float* in, float* out, int64_t n,
+                                              int32_t mant_bits, lof_round_spec spec, float scale,
+                                              const uint32_t* rand_bits, uint64_t seed,
+                                              lof_stream_t stream);
+
+/* MX block rounding over contiguous blocks of block_size elements (the last block may be
+ * short): scale = virtual_round(amax / pmn, *scale_fmt, scale_mode) with
+ * pmn = ldexpf(1, emax) * (2 - 2^-mb) of the element format; out = scale > 0 ?
+ * virtual_round(x / scale, *elem_fmt, elem_spec) * scale : 0.
+ * Replaces lo_float::run_virtual_mx_round (Lof_kernel.cu:70-94, Lof_kernels.h:44-48) and
+ * virtual_mx_round (LoPy_bind.cpp:165-217).  block_size: power of two in [4, 256].
+ * scale_mode: deterministic, or SR-A/B/C/D (which the reference runs with stoch_len 0).
+ * in == out is allowed. */
+LOF_API lof_status lof_virtual_mx_round_f32(const float* in, float* out, int64_t n,
+                                            int32_t block_size, const lof_format* elem_fmt,
+                                            const lof_format* scale_fmt, lof_round_spec elem_spec,
+                                            int32_t scale_mode, const uint32_t* rand_bits,
+                                            uint64_t seed, lof_stream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * Emulated-accumulation GEMM.
+ *
+ * D[i][j] = alpha * acc_ij where, in ascending k,
+ *   chunk == 1 : acc = round(RTO_fma(a_ik, b_kj, acc))   (the reference leaf,
+ *                cutlass_gemms.cuh:139-165: round_to_odd_fma then virtual_round)
+ *   chunk  > 1 : per chunk of `chunk` k values (the last may be short)
+ *                p = fmaf chain of the chunk's products from +0 (FP32, round to nearest),
+ *                acc = round(RTO_fma(p, 1, acc))
+ * starting from acc = +0.  round() is virtual_round(., mant_bits) (kind
+ * LOF_ACCUM_MANTISSA, what the reference's lof_gemm does) or virtual_round(., fmt)
+ * (kind LOF_ACCUM_FORMAT) with the deterministic mode in accum.spec.
+ * RTO_fma(a, b, c) rounds a*b + c to odd in FP32; an exact zero gets the sign
+ * roundTowardNegative gives it.
+ * A: M x K row-major (trans_a = 0, lda >= K) or stored K x M (trans_a = 1, lda >= M).
+ * B: K x N row-major (trans_b = 0, ldb >= N) or stored N x K (trans_b = 1, ldb >= K).
+ * D: M x N row-major, ldd >= N.
+ * ------------------------------------------------------------------------------- */
+enum { LOF_ACCUM_MANTISSA = 0, LOF_ACCUM_FORMAT = 1 };
+
+typedef struct lof_accum {
+    int32_t kind;        /* LOF_ACCUM_* */
+    int32_t mant_bits;   /* LOF_ACCUM_MANTISSA: 0..23 */
+    lof_format fmt;      /* LOF_ACCUM_FORMAT */
+    lof_round_spec spec; /* deterministic mode; the reference kernel always uses RNE */
+    int32_t chunk;       /* >= 1 */
+} lof_accum;
+
+typedef struct lof_gemm_desc {
+    int32_t M, N, K;
+    const float* A;
+    int64_t lda;
+    int32_t trans_a;
+    const float* B;
+    int64_t ldb;
+    int32_t trans_b;
+    float* D;
+    int64_t ldd;
+    float alpha;
+    lof_accum accum;
+} lof_gemm_desc;
+
+/* Run one GEMM.  config: -1 = autotuned choice (or the default tile), else an index
+ * < lof_gemm_num_configs().  Replaces lo_float::LoF_gemm (Lof_kernel.cu:96-154) and
+ * lo_float::Gemm::operator() (cutlass_gemms.cuh:1789-1822; alpha = 1/(scale_a*scale_b)
+ * per run_scaled, 1777-1787, is the caller's job). */
+LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_stream_t stream);
+
+LOF_API int32_t lof_gemm_num_configs(void);
+/* Writes a short "BMxBN/TMxTN" style name; returns LOF_ERR_INVALID_ARGUMENT if out of range. */
+LOF_API lof_status lof_gemm_config_name(int32_t config, char* buf, int32_t buflen);
+
+/* Autotuner: times every tile config on this problem (desc->D is overwritten), records
+ * the winner for problems of the same shape class and returns it. */
+LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
+                                     lof_stream_t stream, int32_t* best_config, float* best_ms);
+
+/* ---------------------------------------------------------------------------------
+ * Grouped persistent launch: many rounding jobs and GEMMs, each with its own format
+ * and mode, in ONE kernel launch that walks a device work queue.
+ * ------------------------------------------------------------------------------- */
+enum {
+    LOF_ITEM_ROUND = 0,      /* lof_virtual_round_f32 semantics */
+    LOF_ITEM_ROUND_MANT = 1, /* lof_virtual_round_mant_f32 semantics */
+    LOF_ITEM_MX = 2,         /* lof_virtual_mx_round_f32 semantics */
+    LOF_ITEM_GEMM = 3        /* lof_gemm_f32 semantics */
+};
+
+typedef struct lof_group_item {
+    int32_t kind;
+    /* rounding items */
+    const float* in;
+    float* out;
+    int64_t n;
+    lof_format fmt;       /* ROUND: target format; MX: element format */
+    int32_t mant_bits;    /* ROUND_MANT */
+    lof_round_spec spec;  /* element rounding */
+    float scale;          /* ROUND / ROUND_MANT pre-scale */
+    const uint32_t* rand_bits;
+    uint64_t seed;
+    int32_t block_size;   /* MX */
+    lof_format scale_fmt; /* MX */
+    int32_t scale_mode;   /* MX */
+    /* GEMM items */
+    lof_gemm_desc gemm;
+} lof_group_item;
+
+typedef struct lof_grouped lof_grouped;
+
+/* Validate the items and upload the work list.  config: -1 = default (or the tuned
+ * choice from lof_grouped_autotune), else < lof_grouped_num_configs(). */
+LOF_API lof_status lof_grouped_create(const lof_group_item* items, int32_t count, int32_t config,
+                                      lof_grouped** plan);
+LOF_API lof_status lof_grouped_run(lof_grouped* plan, lof_stream_t stream);
+/* Launch each item with its own per-item entry point (the baseline the grouped launch
+ * is compared with). */
+LOF_API lof_status lof_grouped_run_separate(lof_grouped* plan, lof_stream_t stream);
+LOF_API lof_status lof_grouped_destroy(lof_grouped* plan);
+LOF_API int32_t lof_grouped_num_configs(void);
+LOF_API lof_status lof_grouped_config_name(int32_t config, char* buf, int32_t buflen);
+/* Total work tiles and persistent grid size of a plan. */
+LOF_API lof_status lof_grouped_info(const lof_grouped* plan, int64_t* tiles, int32_t* grid);
+/* Autotuner for the grouped launch: times every config on these items (outputs are
+ * overwritten) and returns the best. */
+LOF_API lof_status lof_grouped_autotune(const lof_group_item* items, int32_t count, int32_t iters,
+                                        lof_stream_t stream, int32_t* best_config,
+                                        float* best_ms);
+
+/* HIP graph helpers: capture plan runs into a graph and replay it. */
+typedef struct lof_graph lof_graph;
+LOF_API lof_status lof_graph_capture_grouped(lof_grouped* plan, int32_t separate,
+                                             lof_stream_t stream, lof_graph** graph);
+LOF_API lof_status lof_graph_launch(lof_graph* graph, lof_stream_t stream);
+LOF_API lof_status lof_graph_destroy(lof_graph* graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LOFLOAT_H */

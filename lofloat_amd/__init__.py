@@ -1,0 +1,35 @@
+"""lofloat_amd -- MI355X (gfx950) implementation of LoFloat's data-parallel hot path.
+
+Drop-in names of the reference package ``LoFloat`` (LoFloat/__init__.py:1-5):
+``virtual_round``, ``virtual_mx_round``, ``lof_gemm``, ``RoundingMode``, ``InfBehavior``,
+``NaNBehavior``, ``Signedness``, ``FloatFormatDescriptor``, the ``create_*_params``
+factories, the ``torch.ops.lofloat.*`` operators and the layers.  The arithmetic runs in
+hand-written HIP kernels in ``liblofloat_hip.so`` (C-ABI: include/lofloat.h); importing
+this package fails if that library has not been built.
+"""
+from . import _lib  # noqa: F401  (loads liblofloat_hip.so or raises)
+from .core import (FloatFormatDescriptor, InfBehavior, NaNBehavior, RoundingMode,  # noqa: F401
+                   Signedness)
+from .formats import (create_e8m0_params, create_half_params, create_p3109_params,  # noqa: F401
+                      create_single_params, make_format)
+from .ops import (AccumSpec, fill_random_u32, gemm, gemm_autotune,  # noqa: F401
+                  gemm_config_name, gemm_num_configs, lof_gemm, set_seed, virtual_mx_round,
+                  virtual_round)
+from . import _custom_ops  # noqa: F401,E402  (registers torch.ops.lofloat.*)
+from ._custom_ops import get_format, register_format  # noqa: F401,E402
+
+__version__ = "0.1.0"
+
+
+def _out_of_scope(name: str):
+    def f(*args, **kwargs):
+        raise NotImplementedError(
+            f"lofloat_amd.{name}: not part of the accelerated hot path (virtual rounding, MX "
+            "block rounding, emulated-accumulation GEMM, LoF_Linear); use the reference "
+            "LoFloat package for it")
+    f.__name__ = name
+    return f
+
+
+lof_conv2d = _out_of_scope("lof_conv2d")
+pwl_silu = _out_of_scope("pwl_silu")
