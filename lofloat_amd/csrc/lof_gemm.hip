@@ -1,0 +1,230 @@
+// lof_gemm.hip -- emulated-accumulation GEMM launcher, tile-config table, autotuner and
+// C-ABI (lof_gemm_f32, lof_gemm_num_configs, lof_gemm_config_name, lof_gemm_autotune).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "lof_gemm.hpp"
+#include "lof_host.hpp"
+
+namespace lofamd {
+
+namespace {
+
+template <int BM, int BN, int TM, int TN, int BK>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    __shared__ float smem[gemm_smem_floats<BM, BN, BK>()];
+    gemm_tile<BM, BN, TM, TN, BK>(g, (int)blockIdx.y, (int)blockIdx.x, (int)threadIdx.x, smem);
+}
+
+struct GemmConfig {
+    const char* name;
+    int bm, bn;
+    void (*kernel)(GemmArgs);
+};
+
+#define LOF_GEMM_CFG(BM, BN, TM, TN, BK) \
+    {#BM "x" #BN "/" #TM "x" #TN "/k" #BK, BM, BN, k_gemm<BM, BN, TM, TN, BK>}
+
+const GemmConfig kConfigs[] = {
+    LOF_GEMM_CFG(64, 64, 4, 4, 16),
+    LOF_GEMM_CFG(128, 64, 8, 4, 16),
+    LOF_GEMM_CFG(64, 128, 4, 8, 16),
+    LOF_GEMM_CFG(32, 64, 2, 4, 32),
+    LOF_GEMM_CFG(32, 32, 2, 2, 32),
+};
+constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
+
+// autotuned choices, keyed by the problem's shape class
+using TuneKey = std::tuple<int, int, int, int, int, int, int>;
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tuned;
+
+int log2_bucket(int v) {
+    int b = 0;
+    while ((1 << b) < v && b < 30) ++b;
+    return b;
+}
+
+TuneKey tune_key(const lof_gemm_desc& d) {
+    return TuneKey{log2_bucket(d.M), log2_bucket(d.N), log2_bucket(d.K),
+                   d.trans_a * 2 + d.trans_b, d.accum.kind, d.accum.chunk > 1 ? 1 : 0,
+                   d.accum.chunk};
+}
+
+int default_config(const lof_gemm_desc& d) {
+    const int64_t tiles64 = (int64_t)((d.M + 63) / 64) * ((d.N + 63) / 64);
+    if (tiles64 >= 1024) return 0;       // 64x64: enough workgroups to fill 256 CUs
+    if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 512) return 3;
+    return 4;
+}
+
+}  // namespace
+
+void set_fast_path(GemmArgs* g, bool on, int m) {
+    g->fast = on ? 1 : 0;
+    if (on) {
+        g->rh = 20 - m;
+        g->half_m1 = (1u << (g->rh - 1)) - 1u;
+        g->keep_mask = ~((1u << g->rh) - 1u);
+    } else {  // harmless shift values; range 0 sends every leaf to the general path
+        g->rh = 1;
+        g->half_m1 = 0u;
+        g->keep_mask = ~1u;
+        g->range = 0u;
+    }
+}
+
+lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
+    const char* what = "lof_gemm_f32";
+    if (!d) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: desc is NULL", what);
+    if (d->M < 0 || d->N < 0 || d->K < 0)
+        return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: negative dimension", what);
+    const bool empty = d->M == 0 || d->N == 0;
+    if (!empty) {
+        if (!d->D) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: D is NULL", what);
+        if (d->K > 0 && (!d->A || !d->B))
+            return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: A or B is NULL", what);
+        const int64_t min_lda = d->trans_a ? d->M : d->K;
+        const int64_t min_ldb = d->trans_b ? d->K : d->N;
+        if (d->K > 0 && (d->lda < min_lda || d->ldb < min_ldb))
+            return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: lda/ldb too small", what);
+        if (d->ldd < d->N) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: ldd < N", what);
+    }
+    const lof_accum& ac = d->accum;
+    if (ac.chunk < 1) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: chunk < 1", what);
+    if (!mode_valid(ac.spec.mode))
+        return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: bad accumulator mode", what);
+    if (is_stochastic(ac.spec.mode))
+        return set_error(LOF_ERR_UNSUPPORTED,
+                         "%s: stochastic accumulator rounding is not supported", what);
+    *g = GemmArgs{};
+    int m;
+    if (ac.kind == LOF_ACCUM_MANTISSA) {
+        if (ac.mant_bits < 0 || ac.mant_bits > 23)
+            return set_error(LOF_ERR_UNSUPPORTED, "%s: accum mant_bits %d outside [0, 23]", what,
+                             ac.mant_bits);
+        m = ac.mant_bits;
+        g->acc_kind = 0;
+        g->acc_r = 23 - m;
+        g->lo_bound = (uint32_t)(-126 + 1023) << 20;
+        g->range = ((uint32_t)(127 + 1023) << 20) - g->lo_bound;
+    } else if (ac.kind == LOF_ACCUM_FORMAT) {
+        lof_status s = validate_format(&ac.fmt, what);
+        if (s != LOF_OK) return s;
+        m = ac.fmt.mantissa_bits;
+        g->acc_kind = 1;
+        g->accF = make_fmt_dev(ac.fmt, lof_round_spec{ac.spec.mode, 0, 0});
+        int elo = 1 - ac.fmt.bias, ehi = fmt_max_exp(ac.fmt);
+        if (elo < -126) elo = -126;
+        if (ehi > 127) ehi = 127;
+        if (ehi < elo) ehi = elo;  // empty range: every leaf takes the general path
+        g->lo_bound = (uint32_t)(elo + 1023) << 20;
+        g->range = ((uint32_t)(ehi + 1023) << 20) - g->lo_bound;
+    } else {
+        return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: bad accumulator kind %d", what, ac.kind);
+    }
+    g->acc_mode = ac.spec.mode;
+    g->M = d->M; g->N = d->N; g->K = d->K;
+    g->A = d->A; g->lda = d->lda; g->ta = d->trans_a ? 1 : 0;
+    g->B = d->B; g->ldb = d->ldb; g->tb = d->trans_b ? 1 : 0;
+    g->D = d->D; g->ldd = d->ldd;
+    g->alpha = d->alpha;
+    g->chunk = ac.chunk;
+    set_fast_path(g, ac.spec.mode == RNE && m <= 19, m);
+    return LOF_OK;
+}
+
+lof_status launch_gemm(const GemmArgs& g, int config, hipStream_t st) {
+    if (g.M == 0 || g.N == 0) return LOF_OK;
+    const GemmConfig& c = kConfigs[config];
+    dim3 grid((unsigned)((g.N + c.bn - 1) / c.bn), (unsigned)((g.M + c.bm - 1) / c.bm));
+    if (grid.y > 65535u) return set_error(LOF_ERR_UNSUPPORTED, "lof_gemm_f32: M too large");
+    hipLaunchKernelGGL(c.kernel, grid, dim3(256), 0, st, g);
+    return check_hip(hipGetLastError(), "lof_gemm_f32 launch");
+}
+
+int gemm_num_configs() { return kNumConfigs; }
+int gemm_config_bm(int c) { return kConfigs[c].bm; }
+int gemm_config_bn(int c) { return kConfigs[c].bn; }
+
+int resolve_gemm_config(const lof_gemm_desc& d, int config) {
+    if (config >= 0) return config;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(d));
+    return it != g_tuned.end() ? it->second : default_config(d);
+}
+
+}  // namespace lofamd
+
+using namespace lofamd;
+
+extern "C" {
+
+LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_stream_t stream) {
+    const int cfg_bits = config >= 0 ? (config & 0xFF) : -1;
+    if (cfg_bits >= kNumConfigs)
+        return set_error(LOF_ERR_INVALID_ARGUMENT, "lof_gemm_f32: config %d out of range",
+                         config);
+    GemmArgs g;
+    lof_status s = prepare_gemm(desc, &g);
+    if (s != LOF_OK) return s;
+    // bit 8 of config: force the general leaf path (used to test it against the oracle)
+    if (config >= 0 && (config & 0x100)) set_fast_path(&g, false, 0);
+    return launch_gemm(g, resolve_gemm_config(*desc, cfg_bits), (hipStream_t)stream);
+}
+
+LOF_API int32_t lof_gemm_num_configs(void) { return kNumConfigs; }
+
+LOF_API lof_status lof_gemm_config_name(int32_t config, char* buf, int32_t buflen) {
+    if (config < 0 || config >= kNumConfigs || !buf || buflen <= 0)
+        return set_error(LOF_ERR_INVALID_ARGUMENT, "lof_gemm_config_name: bad arguments");
+    std::snprintf(buf, (size_t)buflen, "%s", kConfigs[config].name);
+    return LOF_OK;
+}
+
+LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
+                                     lof_stream_t stream, int32_t* best_config,
+                                     float* best_ms) {
+    GemmArgs g;
+    lof_status s = prepare_gemm(desc, &g);
+    if (s != LOF_OK) return s;
+    if (iters < 1) iters = 1;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if ((s = check_hip(hipEventCreate(&e0), "autotune event")) != LOF_OK) return s;
+    if ((s = check_hip(hipEventCreate(&e1), "autotune event")) != LOF_OK) {
+        (void)hipEventDestroy(e0);
+        return s;
+    }
+    int best = -1;
+    float best_t = 0.f;
+    for (int c = 0; c < kNumConfigs && s == LOF_OK; ++c) {
+        s = launch_gemm(g, c, st);  // warm-up
+        if (s != LOF_OK) break;
+        (void)hipEventRecord(e0, st);
+        for (int i = 0; i < iters && s == LOF_OK; ++i) s = launch_gemm(g, c, st);
+        (void)hipEventRecord(e1, st);
+        if (s != LOF_OK) break;
+        s = check_hip(hipEventSynchronize(e1), "autotune sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= (float)iters;
+        if (best < 0 || ms < best_t) { best = c; best_t = ms; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (s != LOF_OK) return s;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(*desc)] = best;
+    }
+    if (best_config) *best_config = best;
+    if (best_ms) *best_ms = best_t;
+    return LOF_OK;
+}
+
+}  // extern "C"

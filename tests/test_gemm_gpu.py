@@ -1,0 +1,204 @@
+"""GPU parity: emulated-accumulation GEMM against the oracle, bit-exact.
+
+The oracle (oracle/lof_oracle.c) computes every output as the reference leaf
+acc = round(RTO_fma(a, b, acc)) in ascending k (cutlass_gemms.cuh:139-165), or the k-chunk
+rule for chunk > 1, then D = alpha * acc.  Covered: every tile config, all four operand
+layouts, ragged shapes, accumulator mantissas 0..23 and full formats (5e10m saturating and
+extended, e4m3, bf16-like), chunks 1/5/8/32, the forced general leaf path, non-RNE
+accumulator modes, +-0 / inf / NaN / FP32-subnormal / overflowing inputs, the reference
+lof_gemm entry point with scale_a / scale_b, and sampled outputs of a 4096^3 product.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import assert_bits
+
+pytestmark = pytest.mark.gpu
+
+FP16_SAT = (16, 10, 15, 1, 0, 0)
+FP16_EXT = (16, 10, 15, 0, 0, 0)
+E4M3 = (8, 3, 7, 1, 0, 0)
+BF16_LIKE = (16, 7, 127, 1, 0, 0)
+
+
+def _fmt(t):
+    import lofloat_amd as lof
+    return lof.FloatFormatDescriptor(*t, lof.formats.P3109_InfChecker(t[0]),
+                                     lof.formats.P3109_NaNChecker(t[0]))
+
+
+def _mats(M, N, K, seed, kind="normal"):
+    g = np.random.default_rng(seed)
+    A = (g.standard_normal((M, K)) / np.sqrt(max(K, 1))).astype(np.float32)
+    B = (g.standard_normal((K, N))).astype(np.float32)
+    if kind == "e4m3":  # pre-rounded inputs: exact products, many exact ties
+        A = oracle.virtual_round(A * 16, E4M3).reshape(M, K)
+        B = oracle.virtual_round(B, E4M3).reshape(K, N)
+    elif kind == "special":
+        A.flat[g.choice(A.size, max(1, A.size // 50), replace=False)] = 0.0
+        A.flat[g.choice(A.size, max(1, A.size // 50), replace=False)] = -0.0
+        B.flat[g.choice(B.size, max(1, B.size // 50), replace=False)] = -0.0
+        A.flat[g.choice(A.size, max(1, A.size // 200), replace=False)] = 1e-41
+        B.flat[g.choice(B.size, max(1, B.size // 200), replace=False)] = 3e37
+        A.flat[g.choice(A.size, 1, replace=False)] = np.inf
+        B.flat[g.choice(B.size, 1, replace=False)] = np.nan
+    elif kind == "wide":  # magnitudes across the FP32 range
+        A = (A * np.exp2(g.uniform(-60, 60, A.shape))).astype(np.float32)
+        B = (B * np.exp2(g.uniform(-60, 60, B.shape))).astype(np.float32)
+    return A, B
+
+
+def _dev_operands(A, B, ta, tb, dev):
+    At = torch.from_numpy(np.ascontiguousarray(A.T) if ta else A).to(dev)
+    Bt = torch.from_numpy(np.ascontiguousarray(B.T) if tb else B).to(dev)
+    return (At.t() if ta else At), (Bt.t() if tb else Bt)
+
+
+def _run(lof, A, B, accum, dev, ta=False, tb=False, alpha=1.0, config=-1):
+    Ad, Bd = _dev_operands(A, B, ta, tb, dev)
+    return lof.gemm(Ad, Bd, accum, alpha=alpha, config=config).cpu().numpy()
+
+
+def _ref(A, B, oacc, alpha=1.0):
+    return oracle.gemm(A, B, oacc, alpha=alpha)
+
+
+def _accums(lof, m=10, chunk=1, fmt=None, mode=0):
+    if fmt is None:
+        return lof.AccumSpec(mant_bits=m, chunk=chunk, mode=mode), oracle.accum_mant(m, chunk, mode)
+    return (lof.AccumSpec(fmt=_fmt(fmt), chunk=chunk, mode=mode),
+            oracle.accum_fmt(fmt, chunk, mode))
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (17, 33, 65), (64, 64, 64),
+                                   (100, 70, 130), (130, 257, 47), (31, 129, 200)])
+def test_shapes_all_configs(dev, shape):
+    import lofloat_amd as lof
+    M, N, K = shape
+    A, B = _mats(M, N, K, seed=M * 7 + N)
+    acc, oacc = _accums(lof, 10)
+    ref = _ref(A, B, oacc)
+    for cfg in range(lof.gemm_num_configs()):
+        assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref, f"{shape} config {cfg}")
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+def test_layouts(dev, ta, tb):
+    import lofloat_amd as lof
+    A, B = _mats(96, 80, 112, seed=5)
+    for chunk in (1, 8):
+        acc, oacc = _accums(lof, 10, chunk)
+        ref = _ref(A, B, oacc)
+        for cfg in (0, 3):
+            y = _run(lof, A, B, acc, dev, ta, tb, config=cfg)
+            assert_bits(y, ref, f"ta={ta} tb={tb} chunk {chunk} cfg {cfg}")
+
+
+@pytest.mark.parametrize("m", [0, 1, 3, 7, 10, 15, 19, 20, 21, 22, 23])
+@pytest.mark.parametrize("kind", ["normal", "e4m3", "wide"])
+def test_mantissa_accumulators(dev, m, kind):
+    import lofloat_amd as lof
+    A, B = _mats(64, 72, 150, seed=m, kind=kind)
+    for chunk in (1, 5, 32):
+        acc, oacc = _accums(lof, m, chunk)
+        assert_bits(_run(lof, A, B, acc, dev), _ref(A, B, oacc), f"m={m} {kind} chunk {chunk}")
+
+
+@pytest.mark.parametrize("fmt", [FP16_SAT, FP16_EXT, E4M3, BF16_LIKE])
+@pytest.mark.parametrize("kind", ["normal", "e4m3", "wide"])
+def test_format_accumulators(dev, fmt, kind):
+    import lofloat_amd as lof
+    A, B = _mats(48, 64, 200, seed=fmt[1], kind=kind)
+    for chunk in (1, 8, 32):
+        acc, oacc = _accums(lof, chunk=chunk, fmt=fmt)
+        assert_bits(_run(lof, A, B, acc, dev), _ref(A, B, oacc), f"{fmt} {kind} chunk {chunk}")
+
+
+@pytest.mark.parametrize("kind", ["normal", "e4m3", "special", "wide"])
+def test_general_path_forced(dev, kind):
+    """config | 0x100 sends every leaf through the general (literal RTO) path."""
+    import lofloat_amd as lof
+    A, B = _mats(40, 56, 90, seed=3, kind=kind)
+    for acc, oacc in (_accums(lof, 10), _accums(lof, 7, 8), _accums(lof, fmt=FP16_SAT),
+                      _accums(lof, fmt=FP16_EXT, chunk=32)):
+        ref = _ref(A, B, oacc)
+        assert_bits(_run(lof, A, B, acc, dev, config=0x100), ref, f"general {kind}")
+        assert_bits(_run(lof, A, B, acc, dev, config=0), ref, f"fast {kind}")
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4, 5, 6, 7, 12, 13])
+def test_accumulator_modes(dev, mode):
+    import lofloat_amd as lof
+    A, B = _mats(33, 45, 77, seed=mode, kind="e4m3")
+    for acc, oacc in (_accums(lof, 10, mode=mode), _accums(lof, fmt=FP16_SAT, mode=mode, chunk=4)):
+        assert_bits(_run(lof, A, B, acc, dev), _ref(A, B, oacc), f"mode {mode}")
+
+
+def test_special_values(dev):
+    import lofloat_amd as lof
+    A, B = _mats(70, 66, 129, seed=11, kind="special")
+    for acc, oacc in (_accums(lof, 10), _accums(lof, 23), _accums(lof, fmt=FP16_SAT),
+                      _accums(lof, fmt=FP16_EXT, chunk=8), _accums(lof, 10, chunk=32)):
+        assert_bits(_run(lof, A, B, acc, dev), _ref(A, B, oacc), "special")
+
+
+def test_zero_sign_rule(dev):
+    """An exactly-zero accumulator takes roundTowardNegative's sign (cutlass_gemms.cuh:74-80):
+    (+0)+(+0) stays +0, cancellations and -0 products give -0."""
+    import lofloat_amd as lof
+    A = np.array([[1.0, -1.0], [0.0, 0.0], [-0.0, 0.0], [2.0, 0.0]], dtype=np.float32)
+    B = np.array([[1.0], [1.0]], dtype=np.float32)
+    acc, oacc = _accums(lof, 10)
+    y = _run(lof, A, B, acc, dev)
+    ref = _ref(A, B, oacc)
+    assert_bits(y, ref, "zero signs")
+    assert np.signbit(y[0, 0]) and not np.signbit(y[1, 0]) and np.signbit(y[2, 0])
+
+
+def test_alpha_and_reference_entry(dev):
+    import lofloat_amd as lof
+    A, B = _mats(64, 48, 96, seed=2)
+    acc, oacc = _accums(lof, 10)
+    assert_bits(_run(lof, A, B, acc, dev, alpha=0.25), _ref(A, B, oacc, alpha=0.25), "alpha")
+    Ad, Bd = torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev)
+    # run_scaled: alpha = 1 / (scale_a * scale_b) in float, 1 if the product is 0
+    D = lof.lof_gemm(Ad, Bd, 10, lof.RoundingMode.RoundTowardsZero, 3, 4.0, 0.5).cpu().numpy()
+    assert_bits(D, _ref(A, B, oacc, alpha=np.float32(1) / (np.float32(4.0) * np.float32(0.5))),
+                "scaled")
+    D0 = lof.lof_gemm(Ad, Bd, 10, scale_a=0.0).cpu().numpy()
+    assert_bits(D0, _ref(A, B, oacc), "scale 0 -> alpha 1")
+    D1 = torch.ops.lofloat.lof_gemm(Ad, Bd, 10, 0, 0, 1.0, 1.0).cpu().numpy()
+    assert_bits(D1, _ref(A, B, oacc), "torch.ops.lofloat.lof_gemm")
+
+
+@pytest.mark.parametrize("inputs", ["e4m3", "bf16"])
+@pytest.mark.parametrize("chunk", [1, 8, 32])
+def test_workload_4096_sampled(dev, inputs, chunk):
+    """M = N = K = 4096 with e4m3 or 8e7m pre-rounded N(0, 1/sqrt(K)) inputs and a 5e10m
+    (16,10,15) accumulator; 512 sampled outputs checked against the oracle with full K."""
+    import lofloat_amd as lof
+    n = 4096
+    g = np.random.default_rng(4096 + chunk)
+    fm = E4M3 if inputs == "e4m3" else BF16_LIKE
+    A = oracle.virtual_round((g.standard_normal((n, n)) / 64).astype(np.float32), fm).reshape(n, n)
+    B = oracle.virtual_round((g.standard_normal((n, n)) / 64).astype(np.float32), fm).reshape(n, n)
+    samples = g.choice(n * n, 512, replace=False)
+    for acc, oacc in (_accums(lof, 10, chunk), _accums(lof, fmt=FP16_SAT, chunk=chunk)):
+        y = _run(lof, A, B, acc, dev).ravel()[samples]
+        ref = oracle.gemm(A, B, oacc, samples=samples)
+        assert_bits(y, ref, f"4096^3 {inputs} chunk {chunk}")
+
+
+def test_errors(dev):
+    import lofloat_amd as lof
+    a = torch.ones(4, 5, device=dev)
+    with pytest.raises(RuntimeError):
+        lof.lof_gemm(a, torch.ones(4, 3, device=dev), 10)
+    with pytest.raises(NotImplementedError):
+        lof.gemm(a, torch.ones(5, 3, device=dev), lof.AccumSpec(mant_bits=24))
+    with pytest.raises(NotImplementedError):
+        lof.gemm(a, torch.ones(5, 3, device=dev), lof.AccumSpec(mant_bits=10, mode=3))
+    with pytest.raises(ValueError):
+        lof.gemm(a, torch.ones(5, 3, device=dev), lof.AccumSpec(mant_bits=10, chunk=0))
