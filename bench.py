@@ -1,0 +1,266 @@
+"""Benchmark of the LoFloat hot path on one MI355X.
+
+Primary line (``value``): MX block rounding of an 8192 x 8192 FP32 matrix, blocks of 32
+along rows, e4m3 elements with an E8M0 shared scale, round-to-nearest-even, in GB/s of
+algorithmic traffic (read 4 B + write 4 B per element).  One step = one rounding pass over
+the matrix, inputs resident in HBM.
+
+Also reported (``extra``): elementwise virtual_round of 2^24 / 2^28 values to (8,3,7); MX
+with e2m1 and with SR-A random words; the emulated-accumulation GEMM at M=N=K=4096
+(TFLOP/s = 2MNK / t) for 10-bit mantissa and 5e10m accumulators at chunks 1/8/32; one
+LoF_Linear forward+backward step (X 8192x4096, W 4096x4096); and the grouped launch of 64
+uneven rounding jobs + 12 GEMMs against per-job launches, both captured in HIP graphs.
+
+The CPU baseline is the C oracle (a restatement of the reference's scalar semantics) run
+with OpenMP on the host cores on the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+VALU_FP32_PEAK_TFLOPS = 157.3
+METRIC = ("round GB/s (% HBM peak); emulated-accum GEMM TFLOP/s; Linear fwd+bwd step ms")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(fn, iters, warmup=1):
+    import torch
+    for _ in range(warmup):
+        fn(0)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for i in range(iters):
+        fn(i)
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / iters
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the MX kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), if present."""
+    p = os.path.join(REPO, "profiles", "mx_round_8192x8192_e4m3_pmc.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quick", action="store_true", help="primary line only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards with no exchange: see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import lofloat_amd as lof
+
+    dev = torch.device("cuda:0")
+    torch.manual_seed(0)
+    e4m3 = lof.make_format(8, 3, 7)
+    e2m1 = lof.make_format(4, 1, 1)
+    e8m0 = lof.create_e8m0_params()
+
+    # ---------------------------------------------------------------- primary: MX 8192^2
+    R = C = 8192
+    n = R * C
+    g = torch.Generator(device=dev).manual_seed(1)
+    nbuf = 2
+    xs = [torch.randn(R, C, device=dev, generator=g) *
+          torch.exp2(torch.rand(R, 1, device=dev, generator=g) * 16 - 8) for _ in range(nbuf)]
+    ys = [torch.empty_like(xs[0]) for _ in range(nbuf)]
+
+    def mx_step(i):
+        lof.virtual_mx_round(xs[i % nbuf], 32, e4m3, e8m0, out=ys[i % nbuf])
+
+    ms = timed(mx_step, args.steps, args.warmup)
+    bytes_per = 8.0 * n
+    gbs = bytes_per / ms / 1e6
+    log(f"mx e4m3 8192^2: {ms * 1e3:.1f} us/step, {gbs:.0f} GB/s")
+
+    # CPU baseline: the oracle on the host cores, same workload (full matrix)
+    import oracle
+    xh = xs[0].cpu().numpy().ravel()
+    cores = oracle.max_threads()
+    best = None
+    t_end = time.time() + 10.0
+    reps = 0
+    while reps < 3 or (time.time() < t_end and reps < 10):
+        t0 = time.perf_counter()
+        ref = oracle.virtual_mx_round(xh, 32, e4m3.as_tuple(), e8m0.as_tuple(), nthreads=cores)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+        reps += 1
+    cpu_gbs = bytes_per / best / 1e9
+    same = np.array_equal(ys[0].cpu().numpy().ravel().view(np.uint32), ref.view(np.uint32))
+    log(f"cpu oracle: {best * 1e3:.1f} ms ({cpu_gbs:.1f} GB/s, {cores} threads), parity {same}")
+
+    result = {
+        "metric": METRIC,
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic: seeded N(0,1) * 2^U(-8,8) per row",
+        "config": {"workload": "MX block round 8192x8192 FP32, bs 32 along rows, e4m3 "
+                               "(8,3,7) elements, E8M0 scale, RNE", "rows": R, "cols": C,
+                   "block_size": 32, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(),
+                     "kernel": "lofamd::k_mx<RNE,0,2>", "algorithmic_bytes": int(bytes_per)},
+        "cpu_baseline": {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": cores,
+                         "kind": "port",
+                         "sample": f"full 8192x8192 matrix, best of {reps} runs of the C oracle "
+                                   "(OpenMP)", "parity_bitexact": bool(same)},
+    }
+    del xs, ys
+    torch.cuda.empty_cache()
+    if not args.quick:
+        result["extra"] = extras(lof, torch, np, oracle, dev)
+    print(json.dumps(result), flush=True)
+
+
+def extras(lof, torch, np, oracle, dev):
+    out = {}
+    e4m3 = lof.make_format(8, 3, 7)
+    e2m1 = lof.make_format(4, 1, 1)
+    e8m0 = lof.create_e8m0_params()
+    fp16 = lof.make_format(16, 10, 15)
+
+    # C1: elementwise virtual_round, rotating buffers past the 256 MB MALL
+    for lg in (24, 28):
+        n = 1 << lg
+        nb = max(2, (1 << 30) // (8 * n) + 1)
+        xs = [torch.randn(n, device=dev) * torch.exp2(torch.rand(n, device=dev) * 24 - 12)
+              for _ in range(nb)]
+        ys = [torch.empty(n, device=dev) for _ in range(nb)]
+        ms = timed(lambda i: lof.virtual_round(xs[i % nb], e4m3, out=ys[i % nb]), 50 if lg == 24 else 10)
+        out[f"virtual_round_2^{lg}_e4m3_GBs"] = round(8 * n / ms / 1e6, 1)
+        log(f"virtual_round 2^{lg}: {ms * 1e3:.1f} us, {8 * n / ms / 1e6:.0f} GB/s")
+        del xs, ys
+    # C2 variants
+    n = 8192 * 8192
+    x = torch.randn(8192, 8192, device=dev)
+    y = torch.empty_like(x)
+    bits = torch.randint(-2**31, 2**31 - 1, (8192, 8192), dtype=torch.int32, device=dev)
+    ms = timed(lambda i: lof.virtual_mx_round(x, 32, e2m1, e8m0, out=y), 10)
+    out["mx_e2m1_GBs"] = round(8 * n / ms / 1e6, 1)
+    ms = timed(lambda i: lof.virtual_mx_round(x, 32, e4m3, e8m0, 3, 0, 8, rand_bits=bits, out=y), 10)
+    out["mx_e4m3_srA8_bits_GBs"] = round(12 * n / ms / 1e6, 1)
+    log(f"mx variants: {out['mx_e2m1_GBs']} / {out['mx_e4m3_srA8_bits_GBs']} GB/s")
+    del x, y, bits
+
+    # C3: emulated GEMM 4096^3 on e4m3 / 8e7m pre-rounded inputs
+    N = 4096
+    flop = 2.0 * N ** 3
+    for inp, fmt in (("e4m3", e4m3), ("8e7m", lof.make_format(16, 7, 127))):
+        A = lof.virtual_round(torch.randn(N, N, device=dev) / 64, fmt)
+        B = lof.virtual_round(torch.randn(N, N, device=dev) / 64, fmt)
+        for name, acc in (("mant10", lambda c: lof.AccumSpec(mant_bits=10, chunk=c)),
+                          ("5e10m", lambda c: lof.AccumSpec(fmt=fp16, chunk=c))):
+            for chunk in (1, 8, 32):
+                spec = acc(chunk)
+                it = 2 if chunk == 1 else 4
+                ms = timed(lambda i: lof.gemm(A, B, spec), it)
+                key = f"gemm4096_{inp}_{name}_c{chunk}_TFLOPs"
+                out[key] = round(flop / ms / 1e9, 3)
+                log(f"{key}: {ms:.2f} ms, {out[key]} TFLOP/s")
+        del A, B
+    out["gemm_roofline_peak_TFLOPs"] = VALU_FP32_PEAK_TFLOPS
+
+    # C4: LoF_Linear forward + backward step
+    M, K, Nout = 8192, 4096, 4096
+    for label, kw in (("5e10m_c32", dict(accum_format=fp16, accum_chunk=32)),
+                      ("mant10_c1", dict(accum_mant_bits=10, accum_chunk=1))):
+        lin = lof.LoF_Linear(K, Nout, act_params=e4m3, weight_params=e4m3, bias=False,
+                             scaling="mx", mx_block_size=32,
+                             grad_spec=lof.LinearGradSpec(lof.make_format(8, 2, 15), 3, 8,
+                                                          seed=5), **kw).to(dev)
+        with torch.no_grad():
+            lin.weight.normal_(0, 0.02)
+        X = (torch.randn(M, K, device=dev) * 0.02).requires_grad_(True)
+        G = torch.randn(M, Nout, device=dev)
+
+        def step(i):
+            X.grad = None
+            lin.weight.grad = None
+            lin(X).backward(G)
+        ms = timed(step, 2)
+        out[f"linear_step_{label}_ms"] = round(ms, 2)
+        out[f"linear_step_{label}_TFLOPs"] = round(3 * 2.0 * M * K * Nout / ms / 1e9, 3)
+        log(f"linear step {label}: {ms:.1f} ms")
+        del lin, X, G
+    torch.cuda.empty_cache()
+
+    # C5: grouped launch vs per-job launches (both HIP-graph captured)
+    rng = np.random.default_rng(5)
+    jobs = []
+    modes = [0, 12, 4, 1, 3]
+    for i in range(64):
+        nn = int(np.exp2(rng.uniform(12, 24)))
+        e, m = int(rng.integers(2, 9)), int(rng.integers(1, 11))
+        f = lof.make_format(1 + e + m, m, (1 << (e - 1)) - 1)
+        mode = modes[i % 5]
+        xi = torch.randn(nn, device=dev)
+        bt = (torch.randint(-2**31, 2**31 - 1, (nn,), dtype=torch.int32, device=dev)
+              if mode == 3 else None)
+        jobs.append(lof.RoundJob(xi, f, mode, min(8, 23 - m) if mode == 3 else 0, rand_bits=bt))
+    dims = [1024, 2048, 4096, 8192]
+    gemm_flop = 0.0
+    for j in range(12):
+        Mg, Ng, Kg = (int(rng.choice(dims)) for _ in range(3))
+        A = torch.randn(Mg, Kg, device=dev) / 64
+        B = torch.randn(Kg, Ng, device=dev) / 64
+        jobs.append(lof.GemmJob(A, B, lof.AccumSpec(fmt=fp16, chunk=32)))
+        gemm_flop += 2.0 * Mg * Ng * Kg
+    cfg, tune_ms = lof.grouped.autotune(jobs, iters=1)
+    plan = lof.GroupedPlan(jobs)
+    gg = plan.capture_graph(separate=False)
+    gs = plan.capture_graph(separate=True)
+    ms_g = timed(lambda i: gg.replay(), 3)
+    outs_g = [j.out.clone() for j in jobs]
+    ms_s = timed(lambda i: gs.replay(), 3)
+    same = all(torch.equal(a, j.out) for a, j in zip(outs_g, jobs))
+    out.update({"grouped_ms": round(ms_g, 2), "separate_ms": round(ms_s, 2),
+                "grouped_speedup": round(ms_s / ms_g, 3), "grouped_config": cfg,
+                "grouped_bitexact_vs_separate": bool(same),
+                "grouped_round_elems": int(sum(j.input.numel() for j in jobs[:64])),
+                "grouped_gemm_TFLOP": round(gemm_flop / 1e12, 3)})
+    log(f"grouped {ms_g:.1f} ms vs separate {ms_s:.1f} ms, bit-exact {same}")
+    return out
+
+
+if __name__ == "__main__":
+    main()

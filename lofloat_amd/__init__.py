@@ -17,6 +17,12 @@ from .ops import (AccumSpec, fill_random_u32, gemm, gemm_autotune,  # noqa: F401
                   virtual_round)
 from . import _custom_ops  # noqa: F401,E402  (registers torch.ops.lofloat.*)
 from ._custom_ops import get_format, register_format  # noqa: F401,E402
+from .layers import (ExplicitMultiheadAttention, FISRBatchNorm, L1BatchNorm,  # noqa: F401,E402
+                     LinearGradSpec, LinfBatchNorm, LoF_Conv2d, LoF_Linear, LoF_Quantize,
+                     LoFLinearFunction, PWLBatchNorm, PWLSiLU, STEMXRound, STERound, _fwht,
+                     _quantize, exp_mant_quantize, mantissa_quantize, replace_mha_with_explicit)
+from . import grouped  # noqa: F401,E402
+from .grouped import GemmJob, GroupedPlan, MXJob, RoundJob  # noqa: F401,E402
 
 __version__ = "0.1.0"
 

@@ -1,0 +1,136 @@
+"""GPU checks of the grouped persistent launch and its HIP-graph capture.
+
+One launch runs many rounding / MX / GEMM jobs of uneven size, each with its own format
+and mode; every job's output must equal (bit for bit) both the oracle and the job's own
+single-job kernel.  Graph replays are run several times with eager work in between and
+each replay is checked (a replay must not depend on state a previous launch left).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import assert_bits, c1_values, random_words
+
+pytestmark = pytest.mark.gpu
+
+
+def _fmt(t):
+    import lofloat_amd as lof
+    return lof.FloatFormatDescriptor(*t, lof.formats.P3109_InfChecker(t[0]),
+                                     lof.formats.P3109_NaNChecker(t[0]))
+
+
+def _make_jobs(lof, dev, seed=0, n_round=24, n_gemm=3, max_log2=18):
+    g = np.random.default_rng(seed)
+    jobs, refs = [], []
+    modes = [0, 12, 4, 1, 3]
+    for i in range(n_round):
+        n = int(np.exp2(g.uniform(8, max_log2))) + int(g.integers(0, 7))
+        x = c1_values(n, seed=seed * 1000 + i)
+        e = int(g.integers(2, 9))
+        m = int(g.integers(1, 11))
+        ft = (1 + e + m, m, (1 << (e - 1)) - 1, int(g.integers(0, 2)), 0, 0)
+        mode = modes[i % len(modes)]
+        L = min(8, 23 - m) if mode == 3 else 0
+        bits = random_words(n, seed=i) if mode == 3 else None
+        xt = torch.from_numpy(x).to(dev)
+        bt = None if bits is None else torch.from_numpy(bits.view(np.int32)).to(dev)
+        kind = i % 4
+        if kind == 3:  # mantissa-only job
+            jobs.append(lof.RoundJob(xt, m, mode, L, rand_bits=bt))
+            refs.append(oracle.virtual_round_mant(x, m, mode, L, bits=bits))
+        elif kind == 2 and n >= 64:  # MX job
+            bs = 32
+            ef = (8, 3, 7, 1, 0, 0)
+            jobs.append(lof.MXJob(xt, bs, _fmt(ef), lof.create_e8m0_params(), mode, 0, L,
+                                  rand_bits=bt))
+            refs.append(oracle.virtual_mx_round(x, bs, ef, (8, 0, 127, 0, 0, 1), mode, 0, L,
+                                                bits=bits))
+        else:
+            sc = 1.0 if i % 5 else 2.5
+            jobs.append(lof.RoundJob(xt, _fmt(ft), mode, L, scale=sc, rand_bits=bt))
+            refs.append(oracle.virtual_round(x, ft, mode, L, scale=sc, bits=bits))
+    for j in range(n_gemm):
+        M, N, K = (int(v) for v in g.choice([64, 96, 128, 200], 3))
+        A = (g.standard_normal((M, K)) / 8).astype(np.float32)
+        B = (g.standard_normal((K, N)) / 8).astype(np.float32)
+        chunk = [1, 8, 32][j % 3]
+        acc = lof.AccumSpec(mant_bits=10, chunk=chunk) if j % 2 == 0 else \
+            lof.AccumSpec(fmt=_fmt((16, 10, 15, 1, 0, 0)), chunk=chunk)
+        oacc = oracle.accum_mant(10, chunk) if j % 2 == 0 else \
+            oracle.accum_fmt((16, 10, 15, 1, 0, 0), chunk)
+        Bd = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev).t() if j == 1 else \
+            torch.from_numpy(B).to(dev)
+        jobs.append(lof.GemmJob(torch.from_numpy(A).to(dev), Bd, acc))
+        refs.append(oracle.gemm(A, B, oacc))
+    return jobs, refs
+
+
+def _check(jobs, refs, what):
+    for k, (j, r) in enumerate(zip(jobs, refs)):
+        assert_bits(j.out.cpu().numpy().ravel(), np.asarray(r).ravel(), f"{what} job {k}")
+
+
+@pytest.mark.parametrize("config", [0, 1, 2, 3])
+def test_grouped_matches_oracle(dev, config):
+    import lofloat_amd as lof
+    jobs, refs = _make_jobs(lof, dev, seed=config)
+    plan = lof.GroupedPlan(jobs, config=config)
+    tiles, grid = plan.info()
+    assert tiles > 0 and grid > 0
+    for rep in range(3):  # repeated launches reuse the self-resetting work queue
+        for j in jobs:
+            j.out.fill_(np.nan)
+        plan.run()
+        torch.cuda.synchronize()
+        _check(jobs, refs, f"grouped cfg {config} rep {rep}")
+
+
+def test_separate_launches_match(dev):
+    import lofloat_amd as lof
+    jobs, refs = _make_jobs(lof, dev, seed=7)
+    plan = lof.GroupedPlan(jobs)
+    plan.run_separate()
+    torch.cuda.synchronize()
+    _check(jobs, refs, "separate")
+
+
+def test_graph_replays(dev):
+    import lofloat_amd as lof
+    jobs, refs = _make_jobs(lof, dev, seed=11)
+    plan = lof.GroupedPlan(jobs)
+    for separate in (False, True):
+        graph = plan.capture_graph(separate=separate)
+        for rep in range(4):
+            for j in jobs:
+                j.out.fill_(-1.0)                       # eager work between replays
+            torch.cuda.synchronize()
+            graph.replay()
+            torch.cuda.synchronize()
+            _check(jobs, refs, f"graph separate={separate} replay {rep}")
+
+
+def test_autotune_and_empty(dev):
+    import lofloat_amd as lof
+    jobs, refs = _make_jobs(lof, dev, seed=3, n_round=8, n_gemm=1)
+    cfg, ms = lof.grouped.autotune(jobs, iters=2)
+    assert 0 <= cfg < lof.grouped.num_configs() and ms > 0
+    plan = lof.GroupedPlan(jobs)  # uses the tuned default
+    plan.run()
+    torch.cuda.synchronize()
+    _check(jobs, refs, "after autotune")
+    empty = lof.GroupedPlan([lof.RoundJob(torch.empty(0, device=dev), 10)])
+    assert empty.info()[0] == 0
+    empty.run()
+
+
+def test_gemm_autotune(dev):
+    import lofloat_amd as lof
+    A = torch.randn(256, 192, device=dev)
+    B = torch.randn(192, 320, device=dev)
+    acc = lof.AccumSpec(mant_bits=10, chunk=8)
+    cfg, ms = lof.gemm_autotune(A, B, acc, iters=2)
+    assert 0 <= cfg < lof.gemm_num_configs() and ms > 0
+    ref = lof.gemm(A, B, acc, config=0)
+    assert torch.equal(lof.gemm(A, B, acc), ref)  # every config gives the same bits
