@@ -45,11 +45,13 @@ struct GroupTask {
 
 constexpr int kRVPT = 2;  // float4 slots per lane per sub-tile
 
+static_assert(256 * kRVPT * 4 == kMxTileElems, "rounding and MX sub-tiles are 2048 floats");
+
 template <int BM, int BN, int TM, int TN, int BK, int SUB>
 struct GroupedCfg {
     static constexpr int bm = BM, bn = BN, tm = TM, tn = TN, bk = BK, sub = SUB;
-    // floats per rounding tile: SUB sub-tiles of 256 lanes x kRVPT float4 slots
-    static constexpr int64_t round_tile_elems = (int64_t)SUB * 256 * kRVPT * 4;
+    // floats per rounding / MX tile: SUB sub-tiles of 2048 floats
+    static constexpr int64_t round_tile_elems = (int64_t)SUB * kMxTileElems;
 };
 
 // Each (mode, random source) tile routine is its own non-inlined function: one call per
@@ -62,7 +64,8 @@ __device__ __noinline__ void g_round_tile(const RoundArgs& a, int64_t st, int ti
 
 template <int MODE, int RSRC>
 __device__ __noinline__ void g_mx_tile(const MxArgs& a, int64_t st, int tid) {
-    mx_tile<MODE, RSRC, kRVPT>(a, st, tid);
+    if (a.lanes_log2 == 0) mx_tile<MODE, RSRC, 4, false>(a, st, tid);  // block_size 4
+    else mx_tile<MODE, RSRC, 8, false>(a, st, tid);
 }
 
 template <bool MANT>

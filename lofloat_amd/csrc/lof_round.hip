@@ -4,14 +4,15 @@
 //   lof_virtual_mx_round_f32    MX block rounding with a shared scale per block
 //   lof_fill_random_u32         counter-based random words
 //
-// Layout: one lane owns 4 consecutive floats (one 16-byte global_load_dwordx4 / store),
-// a 256-thread workgroup covers VPT * 1024 floats, so every wave moves 64 * 16 B per
-// access, fully coalesced.  The arithmetic is 32-bit integer work on the bit patterns
+// Layout: elementwise, one lane owns 4 consecutive floats (one 16-byte global_load_dwordx4
+// / store) and a 256-thread workgroup covers 1024 floats, so every wave moves 64 * 16 B per
+// access, fully coalesced; one float4 per lane with one workgroup per 1024 floats measured
+// fastest on gfx950 HBM.  The arithmetic is 32-bit integer work on the bit patterns
 // (lof_core.hpp), which leaves the kernels bound by HBM bandwidth.
 //
-// MX: a block of bs floats is bs/4 consecutive lanes (bs = 4..256 -> 1..64 lanes); block
-// boundaries never straddle a wave, so the block amax is a __shfl_xor butterfly over those
-// lanes and the shared scale lives in registers.
+// MX: one lane owns 8 consecutive floats (4 when block_size is 4) and a block of bs floats
+// is bs/8 consecutive lanes; blocks never straddle a wave, so the block amax is a DPP
+// butterfly over those lanes and the shared scale lives in registers.
 #include <hip/hip_runtime.h>
 
 #include "lof_core.hpp"
@@ -29,9 +30,9 @@ __global__ __launch_bounds__(kTPB) void k_round(RoundArgs a) {
     round_tile<MODE, MANT, RSRC, VPT>(a, (int64_t)blockIdx.x, threadIdx.x);
 }
 
-template <int MODE, int RSRC, int VPT>
+template <int MODE, int RSRC, int EPL, bool SRNE>
 __global__ __launch_bounds__(kTPB) void k_mx(MxArgs a) {
-    mx_tile<MODE, RSRC, VPT>(a, (int64_t)blockIdx.x, threadIdx.x);
+    mx_tile<MODE, RSRC, EPL, SRNE>(a, (int64_t)blockIdx.x, threadIdx.x);
 }
 
 __global__ __launch_bounds__(kTPB) void k_fill_random(uint32_t* bits, int64_t n, uint64_t seed,
@@ -41,8 +42,7 @@ __global__ __launch_bounds__(kTPB) void k_fill_random(uint32_t* bits, int64_t n,
         bits[i] = rand_u32(seed, offset + (uint64_t)i);
 }
 
-constexpr int kRoundVPT = 4;
-constexpr int kMxVPT = 2;
+constexpr int kRoundVPT = 1;  // one float4 per lane: measured best on gfx950 HBM
 
 template <bool MANT, int RSRC>
 lof_status launch_round_mode(const RoundArgs& a, int mode, hipStream_t st) {
@@ -74,15 +74,14 @@ lof_status launch_round_mode(const RoundArgs& a, int mode, hipStream_t st) {
     return check_hip(hipGetLastError(), "virtual_round launch");
 }
 
-template <int RSRC>
+template <int RSRC, int EPL, bool SRNE>
 lof_status launch_mx_mode(const MxArgs& a, int mode, hipStream_t st) {
-    const int64_t per_block = (int64_t)kTPB * kMxVPT * 4;
-    const int64_t blocks = (a.n + per_block - 1) / per_block;
+    const int64_t blocks = (a.n + kMxTileElems - 1) / kMxTileElems;
     if (blocks > 0x7FFFFFFF) return set_error(LOF_ERR_UNSUPPORTED, "tensor too large");
     dim3 grid((unsigned)blocks), block(kTPB);
-#define LOF_CASE(M)                                                               \
-    case M:                                                                       \
-        hipLaunchKernelGGL((k_mx<M, RSRC, kMxVPT>), grid, block, 0, st, a);       \
+#define LOF_CASE(M)                                                                 \
+    case M:                                                                         \
+        hipLaunchKernelGGL((k_mx<M, RSRC, EPL, SRNE>), grid, block, 0, st, a);      \
         break;
     if constexpr (RSRC == 0) {
         switch (mode) {
@@ -125,10 +124,21 @@ lof_status launch_round(const RoundArgs& a, bool mant, int mode, int rsrc, hipSt
     return launch_round_mode<false, 2>(a, mode, st);
 }
 
+template <int EPL, bool SRNE>
+lof_status launch_mx_epl(const MxArgs& a, int mode, int rsrc, hipStream_t st) {
+    if (rsrc == 0) return launch_mx_mode<0, EPL, SRNE>(a, mode, st);
+    if (rsrc == 1) return launch_mx_mode<1, EPL, SRNE>(a, mode, st);
+    return launch_mx_mode<2, EPL, SRNE>(a, mode, st);
+}
+
 lof_status launch_mx(const MxArgs& a, int mode, int rsrc, hipStream_t st) {
-    if (rsrc == 0) return launch_mx_mode<0>(a, mode, st);
-    if (rsrc == 1) return launch_mx_mode<1>(a, mode, st);
-    return launch_mx_mode<2>(a, mode, st);
+    const bool srne = a.S.mode == RNE;
+    if (a.lanes_log2 == 0) {  // block_size 4: one float4 per lane
+        return srne ? launch_mx_epl<4, true>(a, mode, rsrc, st)
+                    : launch_mx_epl<4, false>(a, mode, rsrc, st);
+    }
+    return srne ? launch_mx_epl<8, true>(a, mode, rsrc, st)
+                : launch_mx_epl<8, false>(a, mode, rsrc, st);
 }
 
 lof_status prepare_round(const float* in, float* out, int64_t n, const lof_format* fmt,

@@ -1,7 +1,8 @@
 // lof_round_kernels.hpp -- per-tile device routines of the rounding kernels.  A "tile" is
-// what one 256-thread workgroup does in one step: VPT * 1024 consecutive floats.  The
-// single-tensor kernels run one tile per workgroup; the grouped persistent kernel pulls
-// tiles of many tensors from a work queue and calls the same routines.
+// what one 256-thread workgroup does in one step: VPT * 1024 consecutive floats for the
+// elementwise rounding, 2048 for MX.  The single-tensor kernels run one tile per
+// workgroup; the grouped persistent kernel pulls tiles of many tensors from a work queue
+// and calls the same routines.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -106,90 +107,119 @@ __device__ __forceinline__ void round_tile(const RoundArgs& a, int64_t tile, int
 // NaN-ignoring max as std::max(amax, v) with amax never NaN (LoPy_bind.cpp:207).
 __device__ __forceinline__ float max_keep(float m, float v) { return (m < v) ? v : m; }
 
-template <int MODE>
-__device__ __forceinline__ float mx_elem(float x, float scale, float inv, bool use_inv,
-                                         const FmtDev& E, uint32_t u) {
-    const float q = use_inv ? x * inv : x / scale;      // identical values when inv is exact
-    const float r = __uint_as_float(vr_fmt<MODE>(__float_as_uint(q), E, u));
-    return r * scale;
+__device__ __forceinline__ float dpp_shuffle(float v, int ctrl) {
+    switch (ctrl) {  // the control word must be a compile-time constant
+    case 0xB1: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    case 0x4E: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    case 0x141: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    default: return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+    }
 }
 
-template <int MODE, int RSRC, int VPT>
-__device__ __forceinline__ void mx_tile(const MxArgs& a, int64_t tile, int tid) {
-    constexpr int TPB = 256;
-    const int64_t nslots = (a.n + 3) >> 2;               // 4-float slots
-    const int64_t v0 = tile * (TPB * VPT) + tid;
-    const int lanes = 1 << a.lanes_log2;
-    float4 x[VPT];
-    uint4 w[VPT];
+// Max over groups of `lanes` consecutive lanes (a power of two <= 64, groups aligned),
+// result in every lane of the group.  Inputs are never NaN (partial maxima of max_keep),
+// so fmaxf is the plain maximum.  Offsets 1/2 use DPP quad permutes, 4 the half-row
+// mirror and 8 the row mirror (each lane meets a lane of the other half of its group);
+// 16 and 32 cross DPP rows and go through ds_bpermute.
+__device__ __forceinline__ float group_max(float m, int lanes) {
+    if (lanes > 1) m = fmaxf(m, dpp_shuffle(m, 0xB1));   // quad_perm [1,0,3,2]
+    if (lanes > 2) m = fmaxf(m, dpp_shuffle(m, 0x4E));   // quad_perm [2,3,0,1]
+    if (lanes > 4) m = fmaxf(m, dpp_shuffle(m, 0x141));  // row_half_mirror
+    if (lanes > 8) m = fmaxf(m, dpp_shuffle(m, 0x140));  // row_mirror
+    if (lanes > 16) m = fmaxf(m, __shfl_xor(m, 16, 64));
+    if (lanes > 32) m = fmaxf(m, __shfl_xor(m, 32, 64));
+    return m;
+}
+
+// Shared MX scale of a block from its amax: virtual_round(amax / pmn, scale format).
+template <bool SRNE>
+__device__ __forceinline__ float mx_scale(float amax, const MxArgs& a) {
+    const uint32_t t = __float_as_uint(amax / a.pmn);   // division, as the CPU binding
+    return __uint_as_float(SRNE ? vr_fmt<RNE>(t, a.S, 0u) : vr_fmt_rt(t, a.S, 0u));
+}
+
+template <int MODE, int EPL>
+__device__ __forceinline__ void mx_round_lane(float (&x)[EPL], const uint32_t (&u)[EPL],
+                                              float scale, const FmtDev& E) {
+    const uint32_t sb = __float_as_uint(scale);
+    const uint32_t sexp = sb >> 23;  // sign is 0 for every scale that reaches here
+    if (!(scale > 0.0f)) {           // zero / NaN scale zeroes the block
 #pragma unroll
-    for (int j = 0; j < VPT; ++j) {
-        const int64_t vi = v0 + (int64_t)j * TPB;
-        if (a.vec_ok) {
-            if (vi < nslots) {
-                x[j] = reinterpret_cast<const float4*>(a.in)[vi];
-                if constexpr (RSRC == 1) w[j] = reinterpret_cast<const uint4*>(a.bits)[vi];
-            } else {
-                x[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        } else {
-            const int64_t i = 4 * vi;
-            x[j].x = i + 0 < a.n ? a.in[i + 0] : 0.f;
-            x[j].y = i + 1 < a.n ? a.in[i + 1] : 0.f;
-            x[j].z = i + 2 < a.n ? a.in[i + 2] : 0.f;
-            x[j].w = i + 3 < a.n ? a.in[i + 3] : 0.f;
-            if constexpr (RSRC == 1) {
-                w[j].x = i + 0 < a.n ? a.bits[i + 0] : 0u;
-                w[j].y = i + 1 < a.n ? a.bits[i + 1] : 0u;
-                w[j].z = i + 2 < a.n ? a.bits[i + 2] : 0u;
-                w[j].w = i + 3 < a.n ? a.bits[i + 3] : 0u;
-            }
-        }
+        for (int e = 0; e < EPL; ++e) x[e] = 0.0f;
+    } else if ((sb & 0x7FFFFFu) == 0u && sexp <= 254u) {
+        // power-of-two scale: its reciprocal is exact, so x * (1/scale) rounds the same
+        // real number as x / scale
+        const float inv = __uint_as_float(sexp <= 253u ? (254u - sexp) << 23 : 0x00400000u);
+#pragma unroll
+        for (int e = 0; e < EPL; ++e)
+            x[e] = __uint_as_float(vr_fmt<MODE>(__float_as_uint(x[e] * inv), E, u[e])) * scale;
+    } else {
+#pragma unroll
+        for (int e = 0; e < EPL; ++e)
+            x[e] = __uint_as_float(vr_fmt<MODE>(__float_as_uint(x[e] / scale), E, u[e])) * scale;
     }
+}
+
+// One MX tile = 2048 consecutive floats.  EPL = floats per lane: 8 (two 16-byte accesses,
+// block_size >= 8) or 4 (block_size 4, two passes).  A block is block_size / EPL
+// consecutive lanes.
+constexpr int64_t kMxTileElems = 2048;
+
+template <int MODE, int RSRC, int EPL, bool SRNE>
+__device__ __forceinline__ void mx_tile(const MxArgs& a, int64_t tile, int tid) {
+    constexpr int PASSES = (int)(kMxTileElems / (256 * EPL));
+    const int lanes = (1 << a.lanes_log2) * 4 / EPL;
 #pragma unroll
-    for (int j = 0; j < VPT; ++j) {
-        const int64_t vi = v0 + (int64_t)j * TPB;
-        // block amax: lane max, then butterfly over the block's lanes (all inside one wave)
-        float m = 0.0f;
-        m = max_keep(m, fabsf(x[j].x));
-        m = max_keep(m, fabsf(x[j].y));
-        m = max_keep(m, fabsf(x[j].z));
-        m = max_keep(m, fabsf(x[j].w));
-        for (int off = 1; off < lanes; off <<= 1) m = max_keep(m, __shfl_xor(m, off, 64));
-        // scale = virtual_round(amax / pmn, scale format, scale mode)
-        const float t = m / a.pmn;
-        const float scale = __uint_as_float(vr_fmt_rt(__float_as_uint(t), a.S, 0u));
-        const uint32_t sb = __float_as_uint(scale);
-        const uint32_t sexp = (sb >> 23) & 0xFFu;
-        // power-of-two scale with an exactly representable reciprocal: x * (1/scale)
-        // rounds the same real number as x / scale
-        const bool use_inv = (sb & 0x7FFFFFu) == 0u && sexp >= 1u && sexp <= 254u;
-        const float inv = use_inv ? 1.0f / scale : 0.0f;
-        if (vi >= nslots) continue;
-        const bool live = scale > 0.0f;
-        uint32_t u0 = 0, u1 = 0, u2 = 0, u3 = 0;
-        if constexpr (RSRC == 1) { u0 = w[j].x; u1 = w[j].y; u2 = w[j].z; u3 = w[j].w; }
+    for (int pass = 0; pass < PASSES; ++pass) {
+        const int64_t e0 = tile * kMxTileElems + (int64_t)pass * (256 * EPL) + (int64_t)tid * EPL;
+        float x[EPL];
+        uint32_t u[EPL];
+        if (a.vec_ok) {  // 16-byte aligned, n % 4 == 0
+            const int64_t nv = a.n >> 2;
+#pragma unroll
+            for (int h = 0; h < EPL / 4; ++h) {
+                const int64_t vi = (e0 >> 2) + h;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint4 w = make_uint4(0u, 0u, 0u, 0u);
+                if (vi < nv) {
+                    v = reinterpret_cast<const float4*>(a.in)[vi];
+                    if constexpr (RSRC == 1) w = reinterpret_cast<const uint4*>(a.bits)[vi];
+                }
+                x[4 * h + 0] = v.x; x[4 * h + 1] = v.y; x[4 * h + 2] = v.z; x[4 * h + 3] = v.w;
+                u[4 * h + 0] = w.x; u[4 * h + 1] = w.y; u[4 * h + 2] = w.z; u[4 * h + 3] = w.w;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) {
+                const int64_t i = e0 + e;
+                x[e] = i < a.n ? a.in[i] : 0.0f;
+                u[e] = 0u;
+                if constexpr (RSRC == 1) u[e] = i < a.n ? a.bits[i] : 0u;
+            }
+        }
         if constexpr (RSRC == 2) {
-            u0 = rand_u32(a.seed, 4 * vi + 0); u1 = rand_u32(a.seed, 4 * vi + 1);
-            u2 = rand_u32(a.seed, 4 * vi + 2); u3 = rand_u32(a.seed, 4 * vi + 3);
+#pragma unroll
+            for (int e = 0; e < EPL; ++e) u[e] = rand_u32(a.seed, (uint64_t)(e0 + e));
         }
-        float4 y;
-        if (live) {
-            y.x = mx_elem<MODE>(x[j].x, scale, inv, use_inv, a.E, u0);
-            y.y = mx_elem<MODE>(x[j].y, scale, inv, use_inv, a.E, u1);
-            y.z = mx_elem<MODE>(x[j].z, scale, inv, use_inv, a.E, u2);
-            y.w = mx_elem<MODE>(x[j].w, scale, inv, use_inv, a.E, u3);
-        } else {
-            y = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        float m = 0.0f;
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) m = max_keep(m, fabsf(x[e]));
+        m = group_max(m, lanes);
+        const float scale = mx_scale<SRNE>(m, a);
+        mx_round_lane<MODE, EPL>(x, u, scale, a.E);
         if (a.vec_ok) {
-            reinterpret_cast<float4*>(a.out)[vi] = y;
+            const int64_t nv = a.n >> 2;
+#pragma unroll
+            for (int h = 0; h < EPL / 4; ++h) {
+                const int64_t vi = (e0 >> 2) + h;
+                if (vi < nv)
+                    reinterpret_cast<float4*>(a.out)[vi] =
+                        make_float4(x[4 * h + 0], x[4 * h + 1], x[4 * h + 2], x[4 * h + 3]);
+            }
         } else {
-            const int64_t i = 4 * vi;
-            if (i + 0 < a.n) a.out[i + 0] = y.x;
-            if (i + 1 < a.n) a.out[i + 1] = y.y;
-            if (i + 2 < a.n) a.out[i + 2] = y.z;
-            if (i + 3 < a.n) a.out[i + 3] = y.w;
+#pragma unroll
+            for (int e = 0; e < EPL; ++e)
+                if (e0 + e < a.n) a.out[e0 + e] = x[e];
         }
     }
 }
