@@ -127,7 +127,8 @@ LOF_API lof_status lof_virtual_mx_round_f32(const float* in, float* out, int64_t
 /* ---------------------------------------------------------------------------------
  * Emulated-accumulation GEMM.
  *
- * D[i][j] = alpha * acc_ij where, in ascending k,
+ * D[i][j] = alpha * acc_ij (or fmaf(alpha, acc_ij, beta * C[i][j]) when beta != 0, the
+ * CUTLASS LinearCombination epilogue) where, in ascending k,
  *   chunk == 1 : acc = round(RTO_fma(a_ik, b_kj, acc))   (the reference leaf,
  *                cutlass_gemms.cuh:139-165: round_to_odd_fma then virtual_round)
  *   chunk  > 1 : per chunk of `chunk` k values (the last may be short)
@@ -140,7 +141,8 @@ LOF_API lof_status lof_virtual_mx_round_f32(const float* in, float* out, int64_t
  * roundTowardNegative gives it.
  * A: M x K row-major (trans_a = 0, lda >= K) or stored K x M (trans_a = 1, lda >= M).
  * B: K x N row-major (trans_b = 0, ldb >= N) or stored N x K (trans_b = 1, ldb >= K).
- * D: M x N row-major, ldd >= N.
+ * D: M x N row-major, ldd >= N.  C: M x N row-major, ldc >= N (read only when beta != 0;
+ * may alias D).
  * ------------------------------------------------------------------------------- */
 enum { LOF_ACCUM_MANTISSA = 0, LOF_ACCUM_FORMAT = 1 };
 
@@ -164,6 +166,9 @@ typedef struct lof_gemm_desc {
     int64_t ldd;
     float alpha;
     lof_accum accum;
+    const float* C;
+    int64_t ldc;
+    float beta;
 } lof_gemm_desc;
 
 /* Run one GEMM.  config: -1 = autotuned choice (or the default tile), else an index

@@ -1,0 +1,300 @@
+// lo_float.hpp -- C++ host API of the MI355X LoFloat hot path, over the C-ABI in
+// lofloat.h (link liblofloat_hip.so).
+//
+// Mirrors the reference's C++ interface for this path (SudhanvaKulkarni123/LoFloat):
+//   enums + ProjSpec + FloatingPointParams      src/fp_tools.hpp:13-124, 187-252
+//   round_mantissa / round_fp_params             src/LoPy_bind.cpp:13-20, Lof_kernel.cu:37-60
+//   run_virtual_mx_round                          src/Lof_kernels.h:44-48, Lof_kernel.cu:70-94
+//   Layout, Matrix<T, idx, Layout>                src/layouts.h:16-19, src/Matrix.h:21-78
+//   Gemm<MatA, MatB, MatC, MatD>                  src/cutlass_gemms.cuh:1629-1896
+// Same names, argument meanings and enum values.  Differences: every launcher takes an
+// optional hipStream_t (default: the legacy default stream, as the reference launches), an
+// optional per-element random-word array and seed for the stochastic modes, and reports
+// failures by throwing lo_float::Error (the reference's launchers report nothing); the
+// Gemm never allocates a workspace and never synchronises the host.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "../lofloat.h"
+
+namespace lo_float {
+
+enum class Rounding_Mode : uint8_t {
+    RoundToNearestEven, RoundTowardsZero, RoundAwayFromZero, StochasticRoundingA,
+    RoundToNearestOdd, RoundDown, RoundUp, RoundTiesToAway, StochasticRoundingB,
+    StochasticRoundingC, True_StochasticRounding, ProbabilisticRounding, RoundToOdd,
+    RoundTiesTowardsZero, StochasticRoundingD
+};
+enum Saturation_Mode : uint8_t { OvfInf, SatFinite, SatPropagate };
+enum Signedness : uint8_t { Signed, Unsigned };
+enum Unsigned_behavior : uint8_t { NegtoZero, NegtoNaN };
+enum Inf_Behaviors : uint8_t { Extended, Saturating };
+enum NaN_Behaviors : uint8_t { _3109, _754 };
+enum Layout : uint8_t { ColMajor = 0, RowMajor = 1 };
+
+struct ProjSpec {
+    Rounding_Mode rounding_mode;
+    Saturation_Mode saturation_mode;
+    int stoch_length;
+    constexpr explicit ProjSpec(Rounding_Mode rm = Rounding_Mode::RoundToNearestEven,
+                                Saturation_Mode sm = Saturation_Mode::OvfInf, int stoch_len = 0)
+        : rounding_mode(rm), saturation_mode(sm), stoch_length(stoch_len) {}
+};
+
+// Placeholder checker type: rounding never consults the Inf/NaN checkers.
+struct NoChecker {};
+
+template <class IsInfFunctor = NoChecker, class IsNaNFunctor = NoChecker>
+struct FloatingPointParams {
+    int bitwidth;
+    int mantissa_bits;
+    int bias;
+    Inf_Behaviors OV_behavior;
+    NaN_Behaviors NA_behavior;
+    Signedness is_signed;
+    IsInfFunctor IsInf;
+    IsNaNFunctor IsNaN;
+    Unsigned_behavior unsigned_behavior;
+
+    constexpr FloatingPointParams(int bw, int mb, int b, Inf_Behaviors ovb, NaN_Behaviors nab,
+                                  Signedness sgn, IsInfFunctor inf = IsInfFunctor{},
+                                  IsNaNFunctor nan = IsNaNFunctor{},
+                                  Unsigned_behavior ub = Unsigned_behavior::NegtoZero)
+        : bitwidth(bw), mantissa_bits(mb), bias(b), OV_behavior(ovb), NA_behavior(nab),
+          is_signed(sgn), IsInf(inf), IsNaN(nan), unsigned_behavior(ub) {}
+};
+
+class Error : public std::runtime_error {
+public:
+    Error(lof_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+    lof_status status;
+};
+
+namespace detail {
+inline void check(lof_status s, const char* what) {
+    if (s != LOF_OK) throw Error(s, std::string(what) + ": " + lof_last_error());
+}
+template <class I, class N>
+inline lof_format to_c(const FloatingPointParams<I, N>& p) {
+    return lof_format{p.bitwidth, p.mantissa_bits, p.bias, (int32_t)p.OV_behavior,
+                      (int32_t)p.NA_behavior, (int32_t)p.is_signed};
+}
+inline lof_round_spec to_c(const ProjSpec& ps) {
+    return lof_round_spec{(int32_t)ps.rounding_mode, (int32_t)ps.saturation_mode,
+                          ps.stoch_length};
+}
+}  // namespace detail
+
+// out[i] = virtual_round(scale * in[i], mantissa_bits, ps)  (Lof_kernel.cu:37-48).
+template <typename T>
+void round_mantissa(const T* in, T* out, int64_t n, int mantissa_bits, ProjSpec ps, T scale,
+                    hipStream_t stream = nullptr, const uint32_t* rand_bits = nullptr,
+                    uint64_t seed = 0) {
+    static_assert(std::is_same_v<T, float>, "the gfx950 path rounds FP32 tensors");
+    detail::check(lof_virtual_round_mant_f32(in, out, n, mantissa_bits, detail::to_c(ps), scale,
+                                             rand_bits, seed, (lof_stream_t)stream),
+                  "round_mantissa");
+}
+
+// inout[i] = virtual_round(scale * inout[i], params, ps)  (Lof_kernel.cu:49-60).
+template <typename T, class I, class N>
+void round_fp_params(T* inout, int64_t n, FloatingPointParams<I, N> params, ProjSpec ps, T scale,
+                     hipStream_t stream = nullptr, const uint32_t* rand_bits = nullptr,
+                     uint64_t seed = 0) {
+    static_assert(std::is_same_v<T, float>, "the gfx950 path rounds FP32 tensors");
+    const lof_format f = detail::to_c(params);
+    detail::check(lof_virtual_round_f32(inout, inout, n, &f, detail::to_c(ps), scale, rand_bits,
+                                        seed, (lof_stream_t)stream),
+                  "round_fp_params");
+}
+
+// In-place MX block rounding (Lof_kernel.cu:70-94); block_size a power of two in [4, 256].
+template <class I1, class N1, class I2, class N2>
+void run_virtual_mx_round(float* inout, int64_t n, int block_size,
+                          FloatingPointParams<I1, N1> params_public,
+                          FloatingPointParams<I2, N2> params_private, ProjSpec ps_public,
+                          ProjSpec ps_private, hipStream_t stream = nullptr,
+                          const uint32_t* rand_bits = nullptr, uint64_t seed = 0) {
+    const lof_format pub = detail::to_c(params_public), priv = detail::to_c(params_private);
+    detail::check(lof_virtual_mx_round_f32(inout, inout, n, block_size, &priv, &pub,
+                                           detail::to_c(ps_private),
+                                           (int32_t)ps_public.rounding_mode, rand_bits, seed,
+                                           (lof_stream_t)stream),
+                  "run_virtual_mx_round");
+}
+
+// Dense matrix view (Matrix.h:21-78): device pointer, m x n, leading dimension ld.
+template <typename T, typename idx = int, Layout L = ColMajor>
+class Matrix {
+public:
+    idx m;
+    idx n;
+    idx ld;
+    T* data;
+    static constexpr Layout layout = L;
+    using scalar_type = T;
+    using idx_type = idx;
+
+    Matrix(T* data_, idx m_, idx n_, idx ld_) : m(m_), n(n_), ld(ld_), data(data_) {}
+    constexpr idx get_idx(idx row, idx col) const {
+        return L == ColMajor ? col * ld + row : row * ld + col;
+    }
+    constexpr idx rows() const { return m; }
+    constexpr idx cols() const { return n; }
+};
+
+// Emulated-accumulation GEMM (cutlass_gemms.cuh:1629-1896): D = alpha * A * B + beta * C
+// with the FP32 accumulator re-rounded to accum_mantissa_bits after every FMA (the
+// reference leaf), in ascending k.  The CUTLASS tile parameters are accepted for source
+// compatibility; tiles are chosen by the library's autotuned table instead.  Extensions:
+// set_accum_format() rounds the accumulator to a full format, set_chunk() accumulates
+// `chunk` products in FP32 between roundings.
+template <typename MatrixA_, typename MatrixB_, typename MatrixC_, typename MatrixD_,
+          int TbM = 128, int TbN = 128, int TbK = 16, int WpM = 32, int WpN = 32, int WpK = 16,
+          int Stages = 3, bool SplitKSerial = false, typename Swizzle = void>
+class Gemm {
+public:
+    using MatrixA = MatrixA_;
+    using MatrixB = MatrixB_;
+    using MatrixC = MatrixC_;
+    using MatrixD = MatrixD_;
+    static_assert(MatrixC::layout == MatrixD::layout,
+                  "lo_float::Gemm requires C and D to share the same layout");
+    static_assert(std::is_same_v<typename MatrixA::scalar_type, float> &&
+                      std::is_same_v<typename MatrixB::scalar_type, float> &&
+                      std::is_same_v<typename MatrixC::scalar_type, float> &&
+                      std::is_same_v<typename MatrixD::scalar_type, float>,
+                  "the gfx950 GEMM works on FP32 matrices");
+
+    enum class Status {
+        kSuccess = 0,
+        kErrorNotSupported = -1,
+        kErrorAllocation = -2,
+        kErrorInitialize = -3,
+        kErrorExecution = -4
+    };
+
+    Gemm(int accum_mantissa_bits = 23, ProjSpec proj_spec = ProjSpec{})
+        : accum_mantissa_bits_(accum_mantissa_bits), proj_spec_(proj_spec) {}
+
+    void set_accum_mantissa_bits(int bits) { accum_mantissa_bits_ = bits; has_fmt_ = false; }
+    void set_proj_spec(ProjSpec spec) { proj_spec_ = spec; }
+    int get_accum_mantissa_bits() const { return accum_mantissa_bits_; }
+    ProjSpec get_proj_spec() const { return proj_spec_; }
+    template <class I, class N>
+    void set_accum_format(FloatingPointParams<I, N> f) { fmt_ = detail::to_c(f); has_fmt_ = true; }
+    void set_chunk(int chunk) { chunk_ = chunk; }
+
+    Status can_implement(const MatrixA& A, const MatrixB& B, const MatrixC& C,
+                         const MatrixD& D) const {
+        if (A.n != B.m || C.m != A.m || C.n != B.n || D.m != A.m || D.n != B.n)
+            return Status::kErrorNotSupported;
+        if (A.m < 0 || A.n < 0 || B.n < 0) return Status::kErrorNotSupported;
+        return Status::kSuccess;
+    }
+
+    static size_t get_workspace_size(const MatrixA&, const MatrixB&, const MatrixC&,
+                                     const MatrixD&, int = 1) {
+        return 0;
+    }
+
+    // D = (A * B) / (scale_a * scale_b); a zero product means 1 (cutlass_gemms.cuh:1777-1787)
+    Status run_scaled(float scale_a, float scale_b, const MatrixA& A, const MatrixB& B,
+                      const MatrixC& C, MatrixD& D, int split_k_slices = 1,
+                      hipStream_t stream = nullptr) const {
+        const float denom = scale_a * scale_b;
+        const float alpha = denom == 0.0f ? 1.0f : 1.0f / denom;
+        return (*this)(alpha, 0.0f, A, B, C, D, split_k_slices, stream);
+    }
+
+    Status operator()(float alpha, float beta, const MatrixA& A, const MatrixB& B,
+                      const MatrixC& C, MatrixD& D, int split_k_slices = 1,
+                      hipStream_t stream = nullptr) const {
+        if (split_k_slices != 1 || can_implement(A, B, C, D) != Status::kSuccess)
+            return Status::kErrorNotSupported;
+        lof_gemm_desc d = make_desc(alpha, beta, A, B, C, D);
+        const lof_status s = lof_gemm_f32(&d, -1, (lof_stream_t)stream);
+        if (s == LOF_ERR_INVALID_ARGUMENT || s == LOF_ERR_UNSUPPORTED)
+            return Status::kErrorNotSupported;
+        return s == LOF_OK ? Status::kSuccess : Status::kErrorExecution;
+    }
+
+    // Average device time of `reps` runs in milliseconds (negative Status on error).
+    float benchmark(float alpha, float beta, const MatrixA& A, const MatrixB& B,
+                    const MatrixC& C, MatrixD& D, int split_k_slices = 1, int reps = 20,
+                    hipStream_t stream = nullptr) const {
+        Status st = (*this)(alpha, beta, A, B, C, D, split_k_slices, stream);
+        if (st != Status::kSuccess) return (float)st;
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess) return (float)Status::kErrorExecution;
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return (float)Status::kErrorExecution;
+        }
+        (void)hipEventRecord(e0, stream);
+        for (int i = 0; i < reps && st == Status::kSuccess; ++i)
+            st = (*this)(alpha, beta, A, B, C, D, split_k_slices, stream);
+        (void)hipEventRecord(e1, stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return st == Status::kSuccess ? ms / (float)reps : (float)st;
+    }
+
+private:
+    int accum_mantissa_bits_;
+    ProjSpec proj_spec_;
+    lof_format fmt_{};
+    bool has_fmt_ = false;
+    int chunk_ = 1;
+
+    lof_accum accum() const {
+        lof_accum a{};
+        a.kind = has_fmt_ ? LOF_ACCUM_FORMAT : LOF_ACCUM_MANTISSA;
+        a.mant_bits = accum_mantissa_bits_;
+        a.fmt = fmt_;
+        // the reference leaf rounds to nearest even whatever proj_spec says
+        // (cutlass_gemms.cuh:122); a full-format accumulator honours the mode
+        a.spec = lof_round_spec{has_fmt_ ? (int32_t)proj_spec_.rounding_mode : 0, 0, 0};
+        a.chunk = chunk_;
+        return a;
+    }
+
+    // A row-major D is computed directly; a column-major D as D^T = B^T * A^T, which has
+    // the same per-output k order.
+    lof_gemm_desc make_desc(float alpha, float beta, const MatrixA& A, const MatrixB& B,
+                            const MatrixC& C, MatrixD& D) const {
+        lof_gemm_desc d{};
+        d.alpha = alpha;
+        d.beta = beta;
+        d.accum = accum();
+        d.K = (int32_t)A.n;
+        d.D = D.data;
+        d.ldd = D.ld;
+        d.C = C.data;
+        d.ldc = C.ld;
+        if constexpr (MatrixD::layout == RowMajor) {
+            d.M = (int32_t)A.m;
+            d.N = (int32_t)B.n;
+            d.A = A.data; d.lda = A.ld; d.trans_a = MatrixA::layout == ColMajor;
+            d.B = B.data; d.ldb = B.ld; d.trans_b = MatrixB::layout == ColMajor;
+        } else {
+            d.M = (int32_t)B.n;
+            d.N = (int32_t)A.m;
+            d.A = B.data; d.lda = B.ld; d.trans_a = MatrixB::layout == RowMajor;
+            d.B = A.data; d.ldb = A.ld; d.trans_b = MatrixA::layout == RowMajor;
+        }
+        return d;
+    }
+};
+
+}  // namespace lo_float

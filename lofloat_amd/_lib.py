@@ -40,7 +40,8 @@ class GemmDesc(ctypes.Structure):
                 ("A", ctypes.c_void_p), ("lda", ctypes.c_int64), ("trans_a", ctypes.c_int32),
                 ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("trans_b", ctypes.c_int32),
                 ("D", ctypes.c_void_p), ("ldd", ctypes.c_int64), ("alpha", ctypes.c_float),
-                ("accum", Accum)]
+                ("accum", Accum), ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64),
+                ("beta", ctypes.c_float)]
 
 
 class GroupItem(ctypes.Structure):

@@ -93,6 +93,9 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
         if (d->K > 0 && (d->lda < min_lda || d->ldb < min_ldb))
             return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: lda/ldb too small", what);
         if (d->ldd < d->N) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: ldd < N", what);
+        if (d->beta != 0.0f && (!d->C || d->ldc < d->N))
+            return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: beta != 0 needs C with ldc >= N",
+                             what);
     }
     const lof_accum& ac = d->accum;
     if (ac.chunk < 1) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: chunk < 1", what);
@@ -133,6 +136,9 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
     g->B = d->B; g->ldb = d->ldb; g->tb = d->trans_b ? 1 : 0;
     g->D = d->D; g->ldd = d->ldd;
     g->alpha = d->alpha;
+    g->beta = d->beta;
+    g->C = d->beta != 0.0f ? d->C : nullptr;
+    g->ldc = d->ldc;
     g->chunk = ac.chunk;
     set_fast_path(g, ac.spec.mode == RNE && m <= 19, m);
     return LOF_OK;

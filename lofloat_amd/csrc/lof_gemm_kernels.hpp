@@ -37,6 +37,9 @@ struct GemmArgs {
     float* D;
     int64_t ldd;
     float alpha;
+    const float* C;      // read when beta != 0
+    int64_t ldc;
+    float beta;
     int32_t acc_kind;    // 0 mantissa-only, 1 full format
     int32_t acc_r;       // mantissa-only: dropped bits 23 - mant_bits
     int32_t acc_mode;
@@ -201,7 +204,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
 #pragma unroll
             for (int j = 0; j < TN; ++j) acc[i][j] = leaf((double)part[i][j], acc[i][j], g);
     }
-    // epilogue: D = alpha * acc (cutlass LinearCombination with beta = 0)
+    // epilogue (cutlass LinearCombination): D = alpha * acc, or with a source term
+    // D = fma(alpha, acc, beta * C)
 #pragma unroll
     for (int i = 0; i < TM; ++i) {
         const int gm = m0 + ty * TM + i;
@@ -209,7 +213,12 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
 #pragma unroll
         for (int j = 0; j < TN; ++j) {
             const int gn = n0 + tx * TN + j;
-            if (gn < g.N) g.D[(int64_t)gm * g.ldd + gn] = g.alpha * (float)acc[i][j];
+            if (gn >= g.N) continue;
+            const float a = (float)acc[i][j];
+            float d;
+            if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * g.C[(int64_t)gm * g.ldc + gn]);
+            else d = g.alpha * a;
+            g.D[(int64_t)gm * g.ldd + gn] = d;
         }
     }
 }

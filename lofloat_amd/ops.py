@@ -191,9 +191,11 @@ def _mat_view(t: torch.Tensor, what: str):
 
 
 def gemm(A: torch.Tensor, B: torch.Tensor, accum: AccumSpec, alpha: float = 1.0,
-         out: torch.Tensor | None = None, config: int = -1) -> torch.Tensor:
-    """D = alpha * A @ B with the emulated accumulator.  A (M x K) and B (K x N) may be
-    row-major or transposed views (no copy is made for either layout)."""
+         out: torch.Tensor | None = None, config: int = -1, C: torch.Tensor | None = None,
+         beta: float = 0.0) -> torch.Tensor:
+    """D = alpha * A @ B (+ beta * C, as fma(alpha, acc, beta * C)) with the emulated
+    accumulator.  A (M x K) and B (K x N) may be row-major or transposed views (no copy is
+    made for either layout)."""
     _require_device_f32(A, "lof_gemm")
     _require_device_f32(B, "lof_gemm")
     if A.dim() != 2 or B.dim() != 2:
@@ -211,8 +213,15 @@ def gemm(A: torch.Tensor, B: torch.Tensor, accum: AccumSpec, alpha: float = 1.0,
         raise ValueError("lof_gemm: out must be a contiguous float32 (M, N) tensor")
     if M == 0 or N == 0:
         return D
+    Cc = None
+    if beta != 0.0:
+        if C is None or C.shape != (M, N):
+            raise ValueError("lof_gemm: beta != 0 needs C of shape (M, N)")
+        _require_device_f32(C, "lof_gemm C")
+        Cc = C.contiguous()
     desc = _lib.GemmDesc(M, N, K, Av.data_ptr(), lda, int(ta), Bv.data_ptr(), ldb, int(tb),
-                         D.data_ptr(), max(N, 1), float(alpha), accum.c_accum())
+                         D.data_ptr(), max(N, 1), float(alpha), accum.c_accum(),
+                         Cc.data_ptr() if Cc is not None else None, max(N, 1), float(beta))
     with torch.cuda.device(A.device):
         st = _lib.lib.lof_gemm_f32(ctypes.byref(desc), int(config), _stream(A))
     _lib.check(st, "lof_gemm")

@@ -78,7 +78,8 @@ def _load() -> ctypes.CDLL:
     lib.orc_rto_fma_dir.argtypes = [ctypes.c_float] * 3
     lib.orc_gemm_f32.restype = None
     lib.orc_gemm_f32.argtypes = [i32, i32, i32, f32p, i64, i32, f32p, i64, i32, f32p, i64,
-                                 ctypes.c_float, P(_Accum), i64p, i64, i32]
+                                 ctypes.c_float, P(_Accum), i64p, i64, i32, f32p, i64,
+                                 ctypes.c_float]
     lib.orc_max_threads.restype = i32
     return lib
 
@@ -196,11 +197,14 @@ def accum_fmt(fmt, chunk: int = 1, mode: int = RNE) -> _Accum:
 
 
 def gemm(A, B, accum: _Accum, trans_a: bool = False, trans_b: bool = False,
-         alpha: float = 1.0, samples=None, nthreads: int = 0) -> np.ndarray:
+         alpha: float = 1.0, samples=None, nthreads: int = 0, C=None,
+         beta: float = 0.0) -> np.ndarray:
     """Emulated-accumulation GEMM oracle.  A is M x K (or K x M if trans_a), B is K x N
-    (or N x K if trans_b).  With samples (flat i*N+j indices) returns just those outputs."""
+    (or N x K if trans_b), C (read when beta != 0) is M x N.  With samples (flat i*N+j
+    indices) returns just those outputs."""
     A = _f32(A)
     B = _f32(B)
+    Cc = _f32(C) if beta != 0.0 else None
     if trans_a:
         K, M = A.shape
     else:
@@ -217,10 +221,11 @@ def gemm(A, B, accum: _Accum, trans_a: bool = False, trans_b: bool = False,
         _lib.orc_gemm_f32(M, N, K, _ptr(A, ctypes.c_float), lda, int(trans_a),
                           _ptr(B, ctypes.c_float), ldb, int(trans_b), _ptr(out, ctypes.c_float),
                           0, float(alpha), ctypes.byref(accum), _ptr(s, ctypes.c_int64), s.size,
-                          nthreads)
+                          nthreads, _ptr(Cc, ctypes.c_float), N, float(beta))
         return out
     out = np.empty((M, N), dtype=np.float32)
     _lib.orc_gemm_f32(M, N, K, _ptr(A, ctypes.c_float), lda, int(trans_a),
                       _ptr(B, ctypes.c_float), ldb, int(trans_b), _ptr(out, ctypes.c_float), N,
-                      float(alpha), ctypes.byref(accum), None, 0, nthreads)
+                      float(alpha), ctypes.byref(accum), None, 0, nthreads,
+                      _ptr(Cc, ctypes.c_float), N, float(beta))
     return out

@@ -316,9 +316,11 @@ static inline float acc_round(float r, const orc_accum* ac) {
  * chunk c > 1 (k-chunk rule): for each chunk [k0, k0+c) (the last may be short)
  *     p = +0;  p = fmaf(a_k, b_k, p) in ascending k (FP32, round-to-nearest)
  *     acc = round(RTO_fma(p, 1, acc))
- * D = alpha * acc  (epilogue alpha, cutlass_gemms.cuh:1777-1787). */
+ * Epilogue (cutlass LinearCombination, cutlass_gemms.cuh:1686-1690): D = alpha * acc, or
+ * with a source term D = alpha * acc + beta * C contracted to fmaf(alpha, acc, beta * C). */
 static float gemm_one(int i, int j, int K, const float* A, int64_t lda, int ta,
-                      const float* B, int64_t ldb, int tb, float alpha, const orc_accum* ac) {
+                      const float* B, int64_t ldb, int tb, float alpha, const orc_accum* ac,
+                      const float* C, int64_t ldc, float beta) {
     float acc = 0.0f;
     const int c = ac->chunk > 0 ? ac->chunk : 1;
     for (int k0 = 0; k0 < K; k0 += c) {
@@ -337,18 +339,19 @@ static float gemm_one(int i, int j, int K, const float* A, int64_t lda, int ta,
             acc = acc_round(orc_rto_fma(p, 1.0f, acc), ac);
         }
     }
+    if (beta != 0.0f) return fmaf(alpha, acc, beta * C[(int64_t)i * ldc + j]);
     return alpha * acc;
 }
 
 void orc_gemm_f32(int M, int N, int K, const float* A, int64_t lda, int trans_a,
                   const float* B, int64_t ldb, int trans_b, float* D, int64_t ldd,
                   float alpha, const orc_accum* ac, const int64_t* samples,
-                  int64_t n_samples, int nthreads) {
+                  int64_t n_samples, int nthreads, const float* C, int64_t ldc, float beta) {
     if (samples) {
         #pragma omp parallel for schedule(dynamic, 4) num_threads(pick_threads(nthreads))
         for (int64_t s = 0; s < n_samples; ++s) {
             const int i = (int)(samples[s] / N), j = (int)(samples[s] % N);
-            D[s] = gemm_one(i, j, K, A, lda, trans_a, B, ldb, trans_b, alpha, ac);
+            D[s] = gemm_one(i, j, K, A, lda, trans_a, B, ldb, trans_b, alpha, ac, C, ldc, beta);
         }
         return;
     }
@@ -356,5 +359,5 @@ void orc_gemm_f32(int M, int N, int K, const float* A, int64_t lda, int trans_a,
     for (int i = 0; i < M; ++i)
         for (int j = 0; j < N; ++j)
             D[(int64_t)i * ldd + j] = gemm_one(i, j, K, A, lda, trans_a, B, ldb, trans_b,
-                                               alpha, ac);
+                                               alpha, ac, C, ldc, beta);
 }

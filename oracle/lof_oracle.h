@@ -87,13 +87,14 @@ void orc_virtual_mx_round(const float* in, float* out, int64_t n, int block_size
 float orc_rto_fma_dir(float a, float b, float c);
 float orc_rto_fma(float a, float b, float c);
 
-/* Emulated-accumulation GEMM.  D = alpha * acc, acc per output as documented in
- * lof_oracle.c.  trans_a: A stored K x M (lda >= M); trans_b: B stored N x K (ldb >= K).
- * samples != NULL: compute only outputs samples[s] = i*N + j, written to D[s]. */
+/* Emulated-accumulation GEMM.  D = alpha * acc (beta == 0) or fmaf(alpha, acc,
+ * beta * C), acc per output as documented in lof_oracle.c.  trans_a: A stored K x M
+ * (lda >= M); trans_b: B stored N x K (ldb >= K).  samples != NULL: compute only outputs
+ * samples[s] = i*N + j, written to D[s]. */
 void orc_gemm_f32(int M, int N, int K, const float* A, int64_t lda, int trans_a,
                   const float* B, int64_t ldb, int trans_b, float* D, int64_t ldd,
                   float alpha, const orc_accum* acc, const int64_t* samples,
-                  int64_t n_samples, int nthreads);
+                  int64_t n_samples, int nthreads, const float* C, int64_t ldc, float beta);
 
 int orc_max_threads(void);
 

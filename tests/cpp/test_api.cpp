@@ -1,0 +1,204 @@
+// test_api.cpp -- C++ API (include/lofloat/lo_float.hpp) against the oracle.
+//
+//   test_api cpu   argument checks that never touch a device
+//   test_api gpu   round_fp_params / round_mantissa / run_virtual_mx_round and Gemm in
+//                  every layout combination, bit-exact against liblof_oracle.so
+// Built by __graft_entry__.build() with g++ (the header needs no HIP compiler).
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "lofloat/lo_float.hpp"
+#include "../../oracle/lof_oracle.h"
+
+using namespace lo_float;
+
+static int failures = 0;
+#define EXPECT(c, ...)                       \
+    do {                                     \
+        if (!(c)) {                          \
+            std::printf("FAIL: " __VA_ARGS__); \
+            std::printf("\n");               \
+            ++failures;                      \
+        }                                    \
+    } while (0)
+
+static bool same_bits(float a, float b) {
+    if (std::isnan(a) && std::isnan(b)) return true;
+    uint32_t x, y;
+    std::memcpy(&x, &a, 4);
+    std::memcpy(&y, &b, 4);
+    return x == y;
+}
+
+static int compare(const std::vector<float>& got, const std::vector<float>& want,
+                   const char* what) {
+    size_t bad = 0, first = 0;
+    for (size_t i = 0; i < got.size(); ++i)
+        if (!same_bits(got[i], want[i])) {
+            if (!bad) first = i;
+            ++bad;
+        }
+    EXPECT(bad == 0, "%s: %zu mismatches, first at %zu (%g vs %g)", what, bad, first,
+           bad ? got[first] : 0.0, bad ? want[first] : 0.0);
+    return (int)bad;
+}
+
+static void cpu_checks() {
+    ProjSpec ps;
+    EXPECT(ps.rounding_mode == Rounding_Mode::RoundToNearestEven && ps.stoch_length == 0,
+           "ProjSpec default");
+    EXPECT((int)Rounding_Mode::RoundToOdd == 12 && (int)Rounding_Mode::StochasticRoundingD == 14,
+           "Rounding_Mode values");
+    FloatingPointParams<> bad(30, 24, 7, Saturating, _3109, Signed);
+    try {
+        round_fp_params<float>(nullptr, 0, bad, ProjSpec{}, 1.0f);
+        EXPECT(false, "mantissa > 23 must throw");
+    } catch (const Error& e) {
+        EXPECT(e.status == LOF_ERR_UNSUPPORTED, "status for mantissa > 23");
+    }
+    using MatR = Matrix<float, int, RowMajor>;
+    MatR A(nullptr, 4, 5, 5), B(nullptr, 6, 3, 3), C(nullptr, 4, 3, 3), D(nullptr, 4, 3, 3);
+    using GemmR = Gemm<MatR, MatR, MatR, MatR>;
+    GemmR g(10);
+    EXPECT(g.can_implement(A, B, C, D) == GemmR::Status::kErrorNotSupported,
+           "shape mismatch rejected");
+    EXPECT(g.get_workspace_size(A, B, C, D) == 0, "no workspace");
+}
+
+template <class T>
+static T* dev_copy(const std::vector<T>& h) {
+    T* d = nullptr;
+    if (hipMalloc(&d, h.size() * sizeof(T) + 16) != hipSuccess) return nullptr;
+    (void)hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+    return d;
+}
+template <class T>
+static std::vector<T> host_copy(const T* d, size_t n) {
+    std::vector<T> h(n);
+    (void)hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost);
+    return h;
+}
+
+// A m x k matrix stored in `layout` with leading dimension ld, and its row-major image.
+struct HostMat {
+    std::vector<float> store, rowmajor;
+    int ld;
+};
+static HostMat make_mat(int m, int k, Layout L, std::mt19937& rng) {
+    std::normal_distribution<float> nd(0.f, 1.f);
+    HostMat h;
+    h.ld = (L == RowMajor ? k : m) + 3;
+    h.store.assign((size_t)h.ld * (L == RowMajor ? m : k), 0.f);
+    h.rowmajor.assign((size_t)m * k, 0.f);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < k; ++j) {
+            const float v = nd(rng) / 8.f;
+            h.rowmajor[(size_t)i * k + j] = v;
+            h.store[L == RowMajor ? (size_t)i * h.ld + j : (size_t)j * h.ld + i] = v;
+        }
+    return h;
+}
+
+template <Layout LA, Layout LB, Layout LD>
+static void gemm_case(std::mt19937& rng, float beta) {
+    const int M = 67, N = 45, K = 93;
+    HostMat a = make_mat(M, K, LA, rng), b = make_mat(K, N, LB, rng), c = make_mat(M, N, LD, rng);
+    float *dA = dev_copy(a.store), *dB = dev_copy(b.store), *dC = dev_copy(c.store);
+    std::vector<float> zero(c.store.size(), 0.f);
+    float* dD = dev_copy(zero);
+    using MA = Matrix<float, int, LA>;
+    using MB = Matrix<float, int, LB>;
+    using MD = Matrix<float, int, LD>;
+    MA A(dA, M, K, a.ld);
+    MB B(dB, K, N, b.ld);
+    MD C(dC, M, N, c.ld), D(dD, M, N, c.ld);
+    using G = Gemm<MA, MB, MD, MD>;
+    G g(10);
+    auto st = g(0.5f, beta, A, B, C, D);
+    EXPECT(st == G::Status::kSuccess, "gemm status");
+    (void)hipDeviceSynchronize();
+    std::vector<float> out = host_copy(dD, c.store.size()), got((size_t)M * N);
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            got[(size_t)i * N + j] = out[LD == RowMajor ? (size_t)i * c.ld + j : (size_t)j * c.ld + i];
+    orc_accum acc{};
+    acc.kind = 0;
+    acc.mant_bits = 10;
+    acc.chunk = 1;
+    std::vector<float> want((size_t)M * N);
+    orc_gemm_f32(M, N, K, a.rowmajor.data(), K, 0, b.rowmajor.data(), N, 0, want.data(), N, 0.5f,
+                 &acc, nullptr, 0, 0, c.rowmajor.data(), N, beta);
+    char what[96];
+    std::snprintf(what, sizeof(what), "Gemm layouts A%d B%d D%d beta %g", (int)LA, (int)LB,
+                  (int)LD, beta);
+    compare(got, want, what);
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dD);
+}
+
+static void gpu_checks() {
+    std::mt19937 rng(1234);
+    const int n = 100003;
+    std::vector<float> x(n);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    std::uniform_real_distribution<float> ud(-12.f, 12.f);
+    for (auto& v : x) v = nd(rng) * std::exp2(ud(rng));
+    x[0] = 0.f; x[1] = -0.f; x[2] = INFINITY; x[3] = NAN; x[4] = 1e-40f;
+
+    // round_fp_params: e4m3 (8,3,7) saturating, RNE, fused scale 2
+    FloatingPointParams<> e4m3(8, 3, 7, Saturating, _3109, Signed);
+    float* d = dev_copy(x);
+    round_fp_params<float>(d, n, e4m3, ProjSpec{}, 2.0f);
+    (void)hipDeviceSynchronize();
+    std::vector<float> want(n);
+    orc_format f{8, 3, 7, 1, 0, 0};
+    orc_virtual_round_fmt_array(x.data(), want.data(), n, &f, orc_spec{0, 0, 0}, 2.0f, nullptr, 0, 0);
+    compare(host_copy(d, n), want, "round_fp_params e4m3");
+
+    // round_mantissa: 10 bits, SR-A with 5 random bits from explicit words
+    std::vector<uint32_t> words(n);
+    for (auto& w : words) w = rng();
+    uint32_t* dw = dev_copy(words);
+    float* dout = dev_copy(x);
+    float* din = dev_copy(x);
+    round_mantissa<float>(din, dout, n, 10, ProjSpec(Rounding_Mode::StochasticRoundingA, OvfInf, 5),
+                          1.0f, nullptr, dw);
+    (void)hipDeviceSynchronize();
+    orc_virtual_round_mant_array(x.data(), want.data(), n, 10, orc_spec{3, 0, 5}, 1.0f,
+                                 words.data(), 0, 0);
+    compare(host_copy(dout, n), want, "round_mantissa SR-A");
+
+    // run_virtual_mx_round: e2m1 elements, E8M0 scale, bs 32, in place
+    FloatingPointParams<> e2m1(4, 1, 1, Saturating, _3109, Signed);
+    FloatingPointParams<> e8m0(8, 0, 127, Extended, _3109, Unsigned);
+    float* dm = dev_copy(x);
+    run_virtual_mx_round(dm, n, 32, e8m0, e2m1, ProjSpec{}, ProjSpec{});
+    (void)hipDeviceSynchronize();
+    orc_format fe{4, 1, 1, 1, 0, 0}, fs{8, 0, 127, 0, 0, 1};
+    orc_virtual_mx_round(x.data(), want.data(), n, 32, &fe, &fs, orc_spec{0, 0, 0}, 0, nullptr, 0, 0);
+    compare(host_copy(dm, n), want, "run_virtual_mx_round e2m1");
+    (void)hipFree(d); (void)hipFree(dw); (void)hipFree(dout); (void)hipFree(din); (void)hipFree(dm);
+
+    for (float beta : {0.0f, 0.75f}) {
+        gemm_case<RowMajor, RowMajor, RowMajor>(rng, beta);
+        gemm_case<ColMajor, RowMajor, RowMajor>(rng, beta);
+        gemm_case<RowMajor, ColMajor, RowMajor>(rng, beta);
+        gemm_case<ColMajor, ColMajor, RowMajor>(rng, beta);
+        gemm_case<RowMajor, RowMajor, ColMajor>(rng, beta);
+        gemm_case<ColMajor, RowMajor, ColMajor>(rng, beta);
+        gemm_case<RowMajor, ColMajor, ColMajor>(rng, beta);
+        gemm_case<ColMajor, ColMajor, ColMajor>(rng, beta);
+    }
+}
+
+int main(int argc, char** argv) {
+    const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
+    cpu_checks();
+    if (gpu) gpu_checks();
+    std::printf("%s checks: %d failure(s)\n", gpu ? "gpu" : "cpu", failures);
+    return failures ? 1 : 0;
+}
