@@ -162,6 +162,8 @@ def test_noncontiguous_input(dev):
     assert_bits(y, ref, "transposed view")
 
 
+@pytest.mark.slow
+@pytest.mark.timeout(600)
 @pytest.mark.parametrize("ft,mode,mant", [((8, 3, 7, 1, 0, 0), 0, None),
                                           (None, 0, 10)])
 def test_exhaustive_all_fp32_patterns(dev, ft, mode, mant):
