@@ -99,6 +99,46 @@ def test_stochastic_internal_generator(dev, inputs, mode):
     assert_bits(y, ref, f"internal generator mode {mode}")
 
 
+def _ideal_p_up(mode, j, L):
+    """Probability that value 1 + j*2^-11 (tail j/2^L of one e4m3 ulp) rounds up, from the
+    reference's ideal probabilities (test_prob_round.cpp:64-67, lo_float.h:815-819) with a
+    uniform random integer R in [0, 2^L) and a fair coin."""
+    t = j / 2.0 ** L
+    if mode in (oracle.SRA, oracle.TRUESR):
+        return t                                  # unbiased: E[result] = x
+    if mode == oracle.PROB:
+        return 0.5                                # up with probability 1/2 for any tail
+    if mode == oracle.SRB:
+        return (j + 1) / 2.0 ** L                 # increment (R + 1): biased by 2^-L
+    if mode == oracle.SRC:                        # increment ceil(R / 2) * 2 units of 2^-L
+        return sum(1 for R in range(2 ** L) if 2 * ((R >> 1) + (R & 1)) + j >= 2 ** L) / 2.0 ** L
+    if mode == oracle.SRD:                        # increment R + coin
+        return sum(1 for R in range(2 ** L) for c in (0, 1) if R + c + j >= 2 ** L) / 2.0 ** (L + 1)
+    raise AssertionError(mode)
+
+
+@pytest.mark.parametrize("mode", STOCH)
+def test_stochastic_expected_values(dev, mode):
+    """Generated words give each stochastic mode its ideal probability of rounding up.
+    1 + j*2^-11 sits between e4m3 neighbours 1 and 1.125 with a tail of j/256 ulp; over
+    2^20 copies the observed frequency of 1.125 must lie within 6 sigma of the ideal."""
+    import lofloat_amd as lof
+    L, copies = 8, 1 << 20
+    js = [1, 37, 128, 200, 255]
+    x = np.repeat(np.array([1 + j * 2.0 ** -11 for j in js], np.float32), copies)
+    xt = torch.from_numpy(x).to(dev)
+    y = lof.virtual_round(xt, _fmt(FORMATS["e4m3_sat"]), mode, L, seed=0xC0FFEE + mode)
+    y = y.cpu().numpy().reshape(len(js), copies)
+    assert np.all((y == 1.0) | (y == 1.125)), "results outside the two neighbours"
+    for row, j in zip(y, js):
+        p = _ideal_p_up(mode, j, L)
+        got = float(np.mean(row == 1.125))
+        tol = 6 * np.sqrt(max(p * (1 - p), 1e-12) / copies) + 1e-9
+        assert abs(got - p) <= tol, (mode, j, got, p)
+        if mode in (oracle.SRA, oracle.TRUESR):   # unbiased in value as well
+            assert abs(float(np.mean(row.astype(np.float64))) - (1 + j * 2.0 ** -11)) <= 0.125 * tol
+
+
 @pytest.mark.parametrize("scale", [2.0, 0.375, 1e-3, 3.0e4, -1.5])
 def test_fused_scale(dev, inputs, scale):
     import lofloat_amd as lof
