@@ -13,7 +13,7 @@ from .core import (FloatFormatDescriptor, InfBehavior, NaNBehavior, RoundingMode
 from .formats import (create_e8m0_params, create_half_params, create_p3109_params,  # noqa: F401
                       create_single_params, make_format)
 from .ops import (AccumSpec, fill_random_u32, gemm, gemm_autotune,  # noqa: F401
-                  gemm_config_name, gemm_num_configs, lof_gemm, set_seed, virtual_mx_round,
+                  gemm_config_name, gemm_config_valid, gemm_num_configs, lof_gemm, set_seed, virtual_mx_round,
                   virtual_round)
 from . import _custom_ops  # noqa: F401,E402  (registers torch.ops.lofloat.*)
 from ._custom_ops import get_format, register_format  # noqa: F401,E402

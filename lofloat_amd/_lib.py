@@ -67,6 +67,7 @@ SIGNATURES = {
                                         _i32, _vp, _u64, _vp]),
     "lof_gemm_f32": (_i32, [_P(GemmDesc), _i32, _vp]),
     "lof_gemm_num_configs": (_i32, []),
+    "lof_gemm_config_valid": (_i32, [_i32, _i32]),
     "lof_gemm_config_name": (_i32, [_i32, ctypes.c_char_p, _i32]),
     "lof_gemm_autotune": (_i32, [_P(GemmDesc), _i32, _vp, _P(_i32), _P(_f32)]),
     "lof_grouped_create": (_i32, [_P(GroupItem), _i32, _i32, _P(_vp)]),

@@ -1,5 +1,6 @@
 // lof_gemm.hip -- emulated-accumulation GEMM launcher, tile-config table, autotuner and
-// C-ABI (lof_gemm_f32, lof_gemm_num_configs, lof_gemm_config_name, lof_gemm_autotune).
+// C-ABI (lof_gemm_f32, lof_gemm_num_configs, lof_gemm_config_valid, lof_gemm_config_name,
+// lof_gemm_autotune).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -8,26 +9,44 @@
 #include <tuple>
 
 #include "lof_gemm.hpp"
+#include "lof_gemm_mfma.hpp"
 #include "lof_host.hpp"
 
 namespace lofamd {
 
 namespace {
 
-template <int BM, int BN, int TM, int TN, int BK>
+template <int BM, int BN, int TM, int TN, int BK, bool SUBN>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     __shared__ float smem[gemm_smem_floats<BM, BN, BK>()];
-    gemm_tile<BM, BN, TM, TN, BK>(g, (int)blockIdx.y, (int)blockIdx.x, (int)threadIdx.x, smem);
+    gemm_tile<BM, BN, TM, TN, BK, SUBN>(g, (int)blockIdx.y, (int)blockIdx.x, (int)threadIdx.x,
+                                        smem);
 }
+
+template <int WMB, int WNB, int CHUNK, bool SUBN>
+__global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g) {
+    __shared__ __align__(16) float smem[MfmaTile<WMB, WNB>::SMEM_FLOATS];
+    gemm_tile_mfma<WMB, WNB, CHUNK, SUBN>(g, (int)blockIdx.y, (int)blockIdx.x, (int)threadIdx.x,
+                                          smem);
+}
+
+using GemmKernel = void (*)(GemmArgs);
 
 struct GemmConfig {
     const char* name;
     int bm, bn;
-    void (*kernel)(GemmArgs);
+    GemmKernel kernel[2];          // VALU tile, any chunk: [subn]
+    GemmKernel mfma[5][2];         // MFMA tile: [log2(chunk) - 1][subn], chunk in {2..32}
 };
 
-#define LOF_GEMM_CFG(BM, BN, TM, TN, BK) \
-    {#BM "x" #BN "/" #TM "x" #TN "/k" #BK, BM, BN, k_gemm<BM, BN, TM, TN, BK>}
+#define LOF_GEMM_CFG(BM, BN, TM, TN, BK)                                                    \
+    {#BM "x" #BN "/" #TM "x" #TN "/k" #BK, BM, BN,                                           \
+     {k_gemm<BM, BN, TM, TN, BK, false>, k_gemm<BM, BN, TM, TN, BK, true>}, {}}
+#define LOF_MFMA_PAIR(WMB, WNB, C) {k_gemm_mfma<WMB, WNB, C, false>, k_gemm_mfma<WMB, WNB, C, true>}
+#define LOF_GEMM_MFMA_CFG(WMB, WNB)                                                         \
+    {"mfma" #WMB #WNB, 64 * WMB, 64 * WNB, {nullptr, nullptr},                               \
+     {LOF_MFMA_PAIR(WMB, WNB, 2), LOF_MFMA_PAIR(WMB, WNB, 4), LOF_MFMA_PAIR(WMB, WNB, 8),     \
+      LOF_MFMA_PAIR(WMB, WNB, 16), LOF_MFMA_PAIR(WMB, WNB, 32)}}
 
 const GemmConfig kConfigs[] = {
     LOF_GEMM_CFG(64, 64, 4, 4, 16),
@@ -35,8 +54,26 @@ const GemmConfig kConfigs[] = {
     LOF_GEMM_CFG(64, 128, 4, 8, 16),
     LOF_GEMM_CFG(32, 64, 2, 4, 32),
     LOF_GEMM_CFG(32, 32, 2, 2, 32),
+    LOF_GEMM_MFMA_CFG(2, 2),
+    LOF_GEMM_MFMA_CFG(2, 1),
+    LOF_GEMM_MFMA_CFG(1, 1),
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
+constexpr int kMfma128x64 = 6, kMfma64 = 7;
+
+bool chunk_fits_mfma(int chunk) { return chunk >= 2 && chunk <= 32 && (32 % chunk) == 0; }
+
+bool is_mfma(int config) { return kConfigs[config].kernel[0] == nullptr; }
+
+bool config_valid(int config, int chunk) {
+    return config >= 0 && config < kNumConfigs && (!is_mfma(config) || chunk_fits_mfma(chunk));
+}
+
+GemmKernel pick_kernel(const GemmConfig& c, const GemmArgs& g) {
+    if (c.kernel[0]) return c.kernel[g.subn ? 1 : 0];
+    const int l2 = 31 - __builtin_clz((unsigned)g.chunk);
+    return c.mfma[l2 - 1][g.subn ? 1 : 0];
+}
 
 // autotuned choices, keyed by the problem's shape class
 using TuneKey = std::tuple<int, int, int, int, int, int, int>;
@@ -55,10 +92,15 @@ TuneKey tune_key(const lof_gemm_desc& d) {
                    d.accum.chunk};
 }
 
+// Untuned choice, from measurements at M = N = K = 4096 (e4m3 inputs): MFMA tiles for
+// chunks they take (128x64 for c <= 8, 64x64 beyond), the 32x64 VALU tile otherwise; the
+// 32x32 tile when there are too few tiles to fill the chip.
 int default_config(const lof_gemm_desc& d) {
-    const int64_t tiles64 = (int64_t)((d.M + 63) / 64) * ((d.N + 63) / 64);
-    if (tiles64 >= 1024) return 0;       // 64x64: enough workgroups to fill 256 CUs
-    if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 512) return 3;
+    if (chunk_fits_mfma(d.accum.chunk)) {
+        const int64_t tiles_128x64 = (int64_t)((d.M + 127) / 128) * ((d.N + 63) / 64);
+        return (d.accum.chunk <= 8 && tiles_128x64 >= 1024) ? kMfma128x64 : kMfma64;
+    }
+    if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 1024) return 3;
     return 4;
 }
 
@@ -70,12 +112,37 @@ void set_fast_path(GemmArgs* g, bool on, int m) {
         g->rh = 20 - m;
         g->half_m1 = (1u << (g->rh - 1)) - 1u;
         g->keep_mask = ~((1u << g->rh) - 1u);
+        g->rf = 23 - m;
+        g->half_m1f = (1u << (g->rf - 1)) - 1u;
+        g->keep_maskf = ~((1u << g->rf) - 1u);
     } else {  // harmless shift values; range 0 sends every leaf to the general path
         g->rh = 1;
         g->half_m1 = 0u;
         g->keep_mask = ~1u;
         g->range = 0u;
+        g->rf = 1;
+        g->half_m1f = 0u;
+        g->keep_maskf = ~1u;
+        g->range_f = 0u;
+        g->subn = 0;
     }
+}
+
+// Fast-path domain on |s|: [2^lo_e, 2^ehi) (double high words and float bits),
+// lo_e = max(elo - m + 1, -126), which reaches into the format's subnormal range (rounding
+// position 20 - m + (elo - E) <= 19 in the double's high word, 23 - m + (elo - E) <= 22 in
+// the float's bits).
+static void set_fast_domain(GemmArgs* g, int elo, int ehi, int m) {
+    int lo_e = elo - m + 1;
+    if (lo_e < -126) lo_e = -126;
+    if (lo_e > elo) lo_e = elo;
+    g->lo_bound = (uint32_t)(lo_e + 1023) << 20;
+    g->range = ((uint32_t)(ehi + 1023) << 20) - g->lo_bound;
+    g->lo_f = (uint32_t)(lo_e + 127) << 23;
+    g->range_f = ((uint32_t)(ehi + 127) << 23) - g->lo_f;
+    g->elo_b = elo + 127;
+    g->elo_d = elo + 1023;
+    g->subn = lo_e < elo ? 1 : 0;
 }
 
 lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
@@ -113,8 +180,7 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
         m = ac.mant_bits;
         g->acc_kind = 0;
         g->acc_r = 23 - m;
-        g->lo_bound = (uint32_t)(-126 + 1023) << 20;
-        g->range = ((uint32_t)(127 + 1023) << 20) - g->lo_bound;
+        set_fast_domain(g, -126, 127, m);
     } else if (ac.kind == LOF_ACCUM_FORMAT) {
         lof_status s = validate_format(&ac.fmt, what);
         if (s != LOF_OK) return s;
@@ -125,8 +191,7 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
         if (elo < -126) elo = -126;
         if (ehi > 127) ehi = 127;
         if (ehi < elo) ehi = elo;  // empty range: every leaf takes the general path
-        g->lo_bound = (uint32_t)(elo + 1023) << 20;
-        g->range = ((uint32_t)(ehi + 1023) << 20) - g->lo_bound;
+        set_fast_domain(g, elo, ehi, m);
     } else {
         return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: bad accumulator kind %d", what, ac.kind);
     }
@@ -146,10 +211,14 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
 
 lof_status launch_gemm(const GemmArgs& g, int config, hipStream_t st) {
     if (g.M == 0 || g.N == 0) return LOF_OK;
+    if (!config_valid(config, g.chunk))
+        return set_error(LOF_ERR_UNSUPPORTED,
+                         "lof_gemm_f32: config %d needs a chunk in {2, 4, 8, 16, 32} (got %d)",
+                         config, g.chunk);
     const GemmConfig& c = kConfigs[config];
     dim3 grid((unsigned)((g.N + c.bn - 1) / c.bn), (unsigned)((g.M + c.bm - 1) / c.bm));
     if (grid.y > 65535u) return set_error(LOF_ERR_UNSUPPORTED, "lof_gemm_f32: M too large");
-    hipLaunchKernelGGL(c.kernel, grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL(pick_kernel(c, g), grid, dim3(256), 0, st, g);
     return check_hip(hipGetLastError(), "lof_gemm_f32 launch");
 }
 
@@ -178,12 +247,20 @@ LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_s
     GemmArgs g;
     lof_status s = prepare_gemm(desc, &g);
     if (s != LOF_OK) return s;
-    // bit 8 of config: force the general leaf path (used to test it against the oracle)
+    // testing bits of config: 0x100 forces the general leaf path; 0x200 runs every chunk of
+    // an MFMA config on the VALU chain; 0x400 runs every chunk on MFMA without the exactness
+    // test (results then differ where a chunk is inexact)
     if (config >= 0 && (config & 0x100)) set_fast_path(&g, false, 0);
+    if (config >= 0 && (config & 0x200)) g.mfma_mode = 1;
+    else if (config >= 0 && (config & 0x400)) g.mfma_mode = 2;
     return launch_gemm(g, resolve_gemm_config(*desc, cfg_bits), (hipStream_t)stream);
 }
 
 LOF_API int32_t lof_gemm_num_configs(void) { return kNumConfigs; }
+
+LOF_API int32_t lof_gemm_config_valid(int32_t config, int32_t chunk) {
+    return config_valid(config, chunk) ? 1 : 0;
+}
 
 LOF_API lof_status lof_gemm_config_name(int32_t config, char* buf, int32_t buflen) {
     if (config < 0 || config >= kNumConfigs || !buf || buflen <= 0)
@@ -209,6 +286,7 @@ LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
     int best = -1;
     float best_t = 0.f;
     for (int c = 0; c < kNumConfigs && s == LOF_OK; ++c) {
+        if (!config_valid(c, g.chunk)) continue;
         s = launch_gemm(g, c, st);  // warm-up
         if (s != LOF_OK) break;
         (void)hipEventRecord(e0, st);

@@ -12,7 +12,9 @@
 //   normal range): RN_m(RTO24(x)) == RN_m(x) because round-to-odd to 24 bits followed by
 //   round-to-nearest to m+1 <= 22 bits is a correct rounding.  RN_m(x) is the round of
 //   s's 52-bit significand at bit 52-m (which sits in the high word for m <= 19), with an
-//   exact tie on s broken by the sign of e.  ~20 VALU ops, no branches taken.
+//   exact tie on s broken by the sign of e.  Below 2^elo the format's grid has the fixed
+//   quantum 2^(elo - m) and the rounding position moves up accordingly.  ~20 VALU ops; the
+//   leaves of a thread's outputs run branch-free and the rare general-path ones afterwards.
 //   General path (any mode, any m, zeros, infinities, NaN, FP32 subnormal or out-of-range
 //   accumulators): build RTO24(x) literally (neighbours rd/ru of x, rz = rd >= 0 ? rd : ru,
 //   OR the inexact bit; an exact zero gets roundTowardNegative's sign) and apply the
@@ -51,6 +53,17 @@ struct GemmArgs {
     int32_t rh;
     uint32_t lo_bound;   // fast path domain on |s|'s high word: [lo_bound, lo_bound + range)
     uint32_t range;      // 0 when the fast path is off
+    // the same fast path on a float partial sum s (chunk leaf of the MFMA tile)
+    int32_t rf;          // 23 - m
+    uint32_t half_m1f;   // (1 << (rf - 1)) - 1
+    uint32_t keep_maskf; // ~((1 << rf) - 1)
+    uint32_t lo_f;       // domain on |s|'s float bits: [lo_f, lo_f + range_f)
+    uint32_t range_f;    // 0 when the fast path is off
+    int32_t elo_b;       // smallest normal exponent of the format, FP32-biased
+    int32_t elo_d;       // the same, FP64-biased
+    int32_t subn;        // the float fast path covers part of the format's subnormal range
+    int32_t mfma_mode;   // MFMA tile: 0 exactness test, 1 every chunk on the VALU chain,
+                         // 2 every chunk on MFMA (testing only: skips the test)
 };
 
 __device__ __forceinline__ uint32_t next_up_bits(uint32_t b) {
@@ -93,29 +106,80 @@ __device__ __noinline__ double leaf_general(double p, double acc, double s, doub
     return (double)__uint_as_float(R);
 }
 
-// One emulated accumulation step with exact product/partial sum p.  The fast-path result
-// is computed unconditionally; lanes outside its domain (or every lane when the fast path
-// is off: range == 0) redo the step on the general path.
-__device__ __forceinline__ double leaf(double p, double acc, const GemmArgs& g) {
+// One emulated accumulation step with exact product/partial sum p, fast path only: returns
+// false (result undefined) where the general path is needed.  With SUBN the rounding
+// position follows the format's fixed subnormal quantum 2^(elo - m) below 2^elo:
+// P = 20 - m + max(0, elo - E) in s's high word, valid while P <= 19, i.e. for
+// |s| >= 2^(elo - m + 1) (the domain's lower bound then).
+template <bool SUBN>
+__device__ __forceinline__ bool leaf_fast(double p, double acc, const GemmArgs& g, double& out) {
     const double s = p + acc;
     const double bb = s - p;
     const double e = (p - (s - bb)) + (acc - bb);
     const uint32_t hi = (uint32_t)__double2hiint(s);
     const uint32_t lo = (uint32_t)__double2loint(s);
-    const uint32_t lsb = (hi >> g.rh) & 1u;
     const uint32_t same_sign = (((uint32_t)__double2hiint(e) ^ hi) >> 31) ^ 1u;
-    const uint32_t up = (e == 0.0) ? lsb : same_sign;
-    const uint32_t c = (lo != 0u) ? 1u : up;
-    const uint32_t hi2 = (hi + g.half_m1 + c) & g.keep_mask;
-    double r = __hiloint2double((int)hi2, 0);
-    const bool ok = ((hi & 0x7FFFFFFFu) - g.lo_bound) < g.range;
-    if (__builtin_expect(!ok, 0)) r = leaf_general(p, acc, s, e, g);
+    uint32_t hi2;
+    if constexpr (SUBN) {
+        const int eb = (int)((hi >> 20) & 0x7FFu);
+        const uint32_t P = (uint32_t)(g.rh + max(g.elo_d - eb, 0));
+        const uint32_t lsb = (hi >> P) & 1u;
+        const uint32_t up = (e == 0.0) ? lsb : same_sign;
+        const uint32_t c = (lo != 0u) ? 1u : up;
+        const uint32_t unit = 1u << P;
+        hi2 = (hi + (unit >> 1) - 1u + c) & (0u - unit);
+    } else {
+        const uint32_t lsb = (hi >> g.rh) & 1u;
+        const uint32_t up = (e == 0.0) ? lsb : same_sign;
+        const uint32_t c = (lo != 0u) ? 1u : up;
+        hi2 = (hi + g.half_m1 + c) & g.keep_mask;
+    }
+    out = __hiloint2double((int)hi2, 0);
+    return ((hi & 0x7FFFFFFFu) - g.lo_bound) < g.range;
+}
+
+__device__ __noinline__ double leaf_slow(double p, double acc, const GemmArgs& g) {
+    const double s = p + acc;
+    const double bb = s - p;
+    const double e = (p - (s - bb)) + (acc - bb);
+    return leaf_general(p, acc, s, e, g);
+}
+
+// One emulated accumulation step (fast path, else the general path).
+__device__ __forceinline__ double leaf(double p, double acc, const GemmArgs& g) {
+    double r;
+    if (__builtin_expect(!leaf_fast<false>(p, acc, g, r), 0)) r = leaf_slow(p, acc, g);
     return r;
+}
+
+// Leaves of N outputs, in groups that are branch-free in the common case: fast results are
+// committed where valid, outputs needing the general path are recorded in a bit mask, and
+// one wave-uniform branch runs the general path for them (acc still holds their old value).
+template <bool SUBN, int N>
+__device__ __forceinline__ void leaves_d(const double* p, double* acc, const GemmArgs& g) {
+    constexpr int G = 8;   // outputs per branch-free group (bounds the live registers)
+    static_assert(N % G == 0 || N < G, "group size");
+#pragma unroll
+    for (int g0 = 0; g0 < N; g0 += G) {
+        uint32_t slow = 0;
+#pragma unroll
+        for (int o = g0; o < g0 + G && o < N; ++o) {
+            double r;
+            const bool ok = leaf_fast<SUBN>(p[o], acc[o], g, r);
+            acc[o] = ok ? r : acc[o];
+            slow |= (uint32_t)(!ok) << (o - g0);
+        }
+        if (__builtin_expect(__any(slow != 0), 0)) {
+#pragma unroll
+            for (int o = g0; o < g0 + G && o < N; ++o)
+                if ((slow >> (o - g0)) & 1u) acc[o] = leaf_slow(p[o], acc[o], g);
+        }
+    }
 }
 
 // One BM x BN output tile with TM x TN outputs per thread (256 threads), K staged through
 // LDS in BK slices.  tile_m / tile_n index the tile grid.
-template <int BM, int BN, int TM, int TN, int BK>
+template <int BM, int BN, int TM, int TN, int BK, bool SUBN>
 __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int tile_n, int tid,
                                           float* smem) {
     static_assert((BM / TM) * (BN / TN) == 256, "256 threads per tile");
@@ -170,7 +234,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
 #pragma unroll
             for (int j = 0; j < TN; ++j) b[j] = Bs[kk * LDB_S + tx * TN + j];
             if (g.chunk == 1) {
-                double ad[TM], bd[TN];
+                double ad[TM], bd[TN], pd[TM * TN];
 #pragma unroll
                 for (int i = 0; i < TM; ++i) ad[i] = (double)a[i];
 #pragma unroll
@@ -178,7 +242,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
 #pragma unroll
-                    for (int j = 0; j < TN; ++j) acc[i][j] = leaf(ad[i] * bd[j], acc[i][j], g);
+                    for (int j = 0; j < TN; ++j) pd[i * TN + j] = ad[i] * bd[j];
+                leaves_d<SUBN, TM * TN>(pd, &acc[0][0], g);
             } else {
 #pragma unroll
                 for (int i = 0; i < TM; ++i)
@@ -186,23 +251,27 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
                     for (int j = 0; j < TN; ++j) part[i][j] = __builtin_fmaf(a[i], b[j], part[i][j]);
                 if (++in_chunk == g.chunk) {
                     in_chunk = 0;
+                    double pd[TM * TN];
 #pragma unroll
                     for (int i = 0; i < TM; ++i)
 #pragma unroll
                         for (int j = 0; j < TN; ++j) {
-                            acc[i][j] = leaf((double)part[i][j], acc[i][j], g);
+                            pd[i * TN + j] = (double)part[i][j];
                             part[i][j] = 0.0f;
                         }
+                    leaves_d<SUBN, TM * TN>(pd, &acc[0][0], g);
                 }
             }
         }
         __syncthreads();
     }
     if (in_chunk != 0) {   // short last chunk
+        double pd[TM * TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-            for (int j = 0; j < TN; ++j) acc[i][j] = leaf((double)part[i][j], acc[i][j], g);
+            for (int j = 0; j < TN; ++j) pd[i * TN + j] = (double)part[i][j];
+        leaves_d<SUBN, TM * TN>(pd, &acc[0][0], g);
     }
     // epilogue (cutlass LinearCombination): D = alpha * acc, or with a source term
     // D = fma(alpha, acc, beta * C)

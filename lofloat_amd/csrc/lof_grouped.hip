@@ -134,8 +134,12 @@ __global__ __launch_bounds__(256) void k_grouped(const GroupTask* __restrict__ t
             grouped_mx(T.x, T.mode, T.rsrc, local, C::sub, tid);
             break;
         default:
-            gemm_tile<C::bm, C::bn, C::tm, C::tn, C::bk>(
-                T.g, (int)(local / T.tiles_n), (int)(local % T.tiles_n), tid, smem);
+            if (T.g.subn)
+                gemm_tile<C::bm, C::bn, C::tm, C::tn, C::bk, true>(
+                    T.g, (int)(local / T.tiles_n), (int)(local % T.tiles_n), tid, smem);
+            else
+                gemm_tile<C::bm, C::bn, C::tm, C::tn, C::bk, false>(
+                    T.g, (int)(local / T.tiles_n), (int)(local % T.tiles_n), tid, smem);
             __syncthreads();  // smem reuse by the next tile
             break;
         }

@@ -246,6 +246,11 @@ def gemm_num_configs() -> int:
     return int(_lib.lib.lof_gemm_num_configs())
 
 
+def gemm_config_valid(i: int, chunk: int) -> bool:
+    """Whether tile config i can run a GEMM with k-chunk ``chunk``."""
+    return bool(_lib.lib.lof_gemm_config_valid(int(i), int(chunk)))
+
+
 def gemm_config_name(i: int) -> str:
     buf = ctypes.create_string_buffer(64)
     _lib.check(_lib.lib.lof_gemm_config_name(int(i), buf, 64), "gemm_config_name")

@@ -4,7 +4,8 @@ The oracle (oracle/lof_oracle.c) computes every output as the reference leaf
 acc = round(RTO_fma(a, b, acc)) in ascending k (cutlass_gemms.cuh:139-165), or the k-chunk
 rule for chunk > 1, then D = alpha * acc.  Covered: every tile config, all four operand
 layouts, ragged shapes, accumulator mantissas 0..23 and full formats (5e10m saturating and
-extended, e4m3, bf16-like), chunks 1/5/8/32, the forced general leaf path, non-RNE
+extended, e4m3, bf16-like), chunks 1/2/4/5/8/16/32, the MFMA chunk-sum configs (exact,
+mixed and inexact slices, the forced VALU chain and forced MFMA), the general leaf path, non-RNE
 accumulator modes, +-0 / inf / NaN / FP32-subnormal / overflowing inputs, the reference
 lof_gemm entry point with scale_a / scale_b, and sampled outputs of a 4096^3 product.
 """
@@ -47,6 +48,16 @@ def _mats(M, N, K, seed, kind="normal"):
     elif kind == "wide":  # magnitudes across the FP32 range
         A = (A * np.exp2(g.uniform(-60, 60, A.shape))).astype(np.float32)
         B = (B * np.exp2(g.uniform(-60, 60, B.shape))).astype(np.float32)
+    elif kind == "tiny":  # partial sums around 2^-14 .. 2^-26: the fp16 subnormal range
+        A = (A * np.float32(2.0 ** -6)).astype(np.float32)
+        B = (B * np.float32(2.0 ** -8)).astype(np.float32)
+    elif kind in ("e4m3_small", "mixed"):  # the GEMM workload's inputs: N(0, 1/64) in e4m3
+        A = oracle.virtual_round((g.standard_normal((M, K)) / 64).astype(np.float32),
+                                 E4M3).reshape(M, K)
+        B = oracle.virtual_round((g.standard_normal((K, N)) / 64).astype(np.float32),
+                                 E4M3).reshape(K, N)
+        if kind == "mixed":  # rows 0-7 scaled by 2^12: tiles holding them fail the exact test
+            A[:8] *= np.float32(4096.0)
     return A, B
 
 
@@ -77,11 +88,53 @@ def _accums(lof, m=10, chunk=1, fmt=None, mode=0):
 def test_shapes_all_configs(dev, shape):
     import lofloat_amd as lof
     M, N, K = shape
-    A, B = _mats(M, N, K, seed=M * 7 + N)
-    acc, oacc = _accums(lof, 10)
-    ref = _ref(A, B, oacc)
-    for cfg in range(lof.gemm_num_configs()):
-        assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref, f"{shape} config {cfg}")
+    for kind, chunk in (("normal", 1), ("normal", 8), ("e4m3_small", 8)):
+        A, B = _mats(M, N, K, seed=M * 7 + N, kind=kind)
+        acc, oacc = _accums(lof, 10, chunk)
+        ref = _ref(A, B, oacc)
+        for cfg in range(lof.gemm_num_configs()):
+            if not lof.gemm_config_valid(cfg, chunk):
+                with pytest.raises(NotImplementedError):
+                    _run(lof, A, B, acc, dev, config=cfg)
+                continue
+            assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref,
+                        f"{shape} {kind} chunk {chunk} config {cfg}")
+
+
+MFMA_CONFIGS = (5, 6, 7)
+
+
+@pytest.mark.parametrize("kind", ["e4m3_small", "mixed", "normal", "special", "e4m3", "tiny"])
+@pytest.mark.parametrize("chunk", [2, 4, 8, 16, 32])
+def test_mfma_chunk_paths(dev, kind, chunk):
+    """MFMA configs: chunk sums on MFMA where the per-slice exactness test passes, on the
+    VALU fmaf chain elsewhere (config | 0x200 forces the chain everywhere); both must equal
+    the oracle bit for bit."""
+    import lofloat_amd as lof
+    A, B = _mats(200, 150, 100, seed=chunk, kind=kind)
+    for acc, oacc in (_accums(lof, 10, chunk), _accums(lof, fmt=FP16_SAT, chunk=chunk)):
+        ref = _ref(A, B, oacc)
+        for cfg in MFMA_CONFIGS:
+            assert lof.gemm_config_valid(cfg, chunk)
+            assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref, f"{kind} c{chunk} cfg {cfg}")
+            assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x200), ref,
+                        f"{kind} c{chunk} cfg {cfg} chain")
+
+
+@pytest.mark.parametrize("cfg", MFMA_CONFIGS)
+def test_mfma_forced(dev, cfg):
+    """config | 0x400 runs every chunk on MFMA without the exactness test: on inputs whose
+    chunk sums are exact the MFMA chunk sums must give the oracle's result (this checks the
+    MFMA operand and result layouts on every tile position).  The automatic choice must
+    match the oracle on general FP32 inputs as well."""
+    import lofloat_amd as lof
+    for acc, oacc in (_accums(lof, 10, 32), _accums(lof, fmt=FP16_SAT, chunk=4)):
+        A, B = _mats(256, 192, 320, seed=cfg, kind="e4m3_small")
+        assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x400), _ref(A, B, oacc),
+                    "forced exact")
+        A, B = _mats(256, 192, 320, seed=cfg, kind="normal")
+        assert_bits(_run(lof, A, B, acc, dev, config=cfg), _ref(A, B, oacc),
+                    "auto on inexact inputs")
 
 
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
@@ -91,7 +144,9 @@ def test_layouts(dev, ta, tb):
     for chunk in (1, 8):
         acc, oacc = _accums(lof, 10, chunk)
         ref = _ref(A, B, oacc)
-        for cfg in (0, 3):
+        for cfg in (0, 3, 5, 7):
+            if not lof.gemm_config_valid(cfg, chunk):
+                continue
             y = _run(lof, A, B, acc, dev, ta, tb, config=cfg)
             assert_bits(y, ref, f"ta={ta} tb={tb} chunk {chunk} cfg {cfg}")
 
@@ -107,11 +162,11 @@ def test_mantissa_accumulators(dev, m, kind):
 
 
 @pytest.mark.parametrize("fmt", [FP16_SAT, FP16_EXT, E4M3, BF16_LIKE])
-@pytest.mark.parametrize("kind", ["normal", "e4m3", "wide"])
+@pytest.mark.parametrize("kind", ["normal", "e4m3", "wide", "tiny"])
 def test_format_accumulators(dev, fmt, kind):
     import lofloat_amd as lof
     A, B = _mats(48, 64, 200, seed=fmt[1], kind=kind)
-    for chunk in (1, 8, 32):
+    for chunk in (1, 5, 8, 32):
         acc, oacc = _accums(lof, chunk=chunk, fmt=fmt)
         assert_bits(_run(lof, A, B, acc, dev), _ref(A, B, oacc), f"{fmt} {kind} chunk {chunk}")
 
@@ -126,6 +181,8 @@ def test_general_path_forced(dev, kind):
         ref = _ref(A, B, oacc)
         assert_bits(_run(lof, A, B, acc, dev, config=0x100), ref, f"general {kind}")
         assert_bits(_run(lof, A, B, acc, dev, config=0), ref, f"fast {kind}")
+        if lof.gemm_config_valid(5, acc.chunk):
+            assert_bits(_run(lof, A, B, acc, dev, config=5 | 0x100), ref, f"mfma general {kind}")
 
 
 @pytest.mark.parametrize("mode", [1, 2, 4, 5, 6, 7, 12, 13])
