@@ -73,6 +73,13 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run N device copies of the 8192x8192 matrix (a plain copy "
                          "kernel of known traffic, to calibrate HBM byte counters)")
+    ap.add_argument("--only", choices=["gemm"], default=None,
+                    help="run one workload alone (for profiling): 'gemm' = the 4096^3 "
+                         "emulated-accumulation GEMM with --chunk / --accum / --inputs")
+    ap.add_argument("--chunk", type=int, default=32)
+    ap.add_argument("--accum", choices=["mant10", "5e10m"], default="5e10m")
+    ap.add_argument("--inputs", choices=["e4m3", "8e7m"], default="e4m3")
+    ap.add_argument("--config", type=int, default=-1, help="GEMM tile config (-1: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -89,6 +96,9 @@ def main():
     e4m3 = lof.make_format(8, 3, 7)
     e2m1 = lof.make_format(4, 1, 1)
     e8m0 = lof.create_e8m0_params()
+    if args.only == "gemm":
+        gemm_only(lof, torch, dev, args)
+        return
 
     # ---------------------------------------------------------------- primary: MX 8192^2
     R = C = 8192
@@ -164,6 +174,28 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def gemm_inputs(lof, torch, dev, inputs, n=4096):
+    fmt = lof.make_format(8, 3, 7) if inputs == "e4m3" else lof.make_format(16, 7, 127)
+    g = torch.Generator(device=dev).manual_seed(3)
+    A = lof.virtual_round(torch.randn(n, n, device=dev, generator=g) / 64, fmt)
+    B = lof.virtual_round(torch.randn(n, n, device=dev, generator=g) / 64, fmt)
+    return A, B
+
+
+def gemm_only(lof, torch, dev, args):
+    """One GEMM workload alone: M = N = K = 4096, inputs N(0, 1/64) pre-rounded to e4m3 or
+    8e7m, accumulator 10-bit mantissa or 5e10m (16,10,15), chunk --chunk."""
+    n = 4096
+    A, B = gemm_inputs(lof, torch, dev, args.inputs, n)
+    spec = (lof.AccumSpec(mant_bits=10, chunk=args.chunk) if args.accum == "mant10"
+            else lof.AccumSpec(fmt=lof.make_format(16, 10, 15), chunk=args.chunk))
+    ms = timed(lambda i: lof.gemm(A, B, spec, config=args.config), args.steps, args.warmup)
+    tf = 2.0 * n ** 3 / ms / 1e9
+    print(json.dumps({"workload": f"gemm {n}^3 {args.inputs} inputs, {args.accum} accumulator, "
+                                  f"chunk {args.chunk}", "config": args.config,
+                      "ms": round(ms, 3), "TFLOPs": round(tf, 3)}), flush=True)
+
+
 def extras(lof, torch, np, oracle, dev):
     out = {}
     e4m3 = lof.make_format(8, 3, 7)
@@ -197,9 +229,8 @@ def extras(lof, torch, np, oracle, dev):
     # C3: emulated GEMM 4096^3 on e4m3 / 8e7m pre-rounded inputs
     N = 4096
     flop = 2.0 * N ** 3
-    for inp, fmt in (("e4m3", e4m3), ("8e7m", lof.make_format(16, 7, 127))):
-        A = lof.virtual_round(torch.randn(N, N, device=dev) / 64, fmt)
-        B = lof.virtual_round(torch.randn(N, N, device=dev) / 64, fmt)
+    for inp in ("e4m3", "8e7m"):
+        A, B = gemm_inputs(lof, torch, dev, inp, N)
         for name, acc in (("mant10", lambda c: lof.AccumSpec(mant_bits=10, chunk=c)),
                           ("5e10m", lambda c: lof.AccumSpec(fmt=fp16, chunk=c))):
             for chunk in (1, 8, 32):

@@ -93,12 +93,13 @@ TuneKey tune_key(const lof_gemm_desc& d) {
 }
 
 // Untuned choice, from measurements at M = N = K = 4096 (e4m3 inputs): MFMA tiles for
-// chunks they take (128x64 for c <= 8, 64x64 beyond), the 32x64 VALU tile otherwise; the
-// 32x32 tile when there are too few tiles to fill the chip.
+// chunks they take (64x64 for c <= 8, where the leaves dominate; 128x64 for c >= 16 when
+// that still gives >= 1024 tiles), the 32x64 VALU tile otherwise; the 32x32 VALU tile when
+// there are too few tiles to fill the chip.
 int default_config(const lof_gemm_desc& d) {
     if (chunk_fits_mfma(d.accum.chunk)) {
         const int64_t tiles_128x64 = (int64_t)((d.M + 127) / 128) * ((d.N + 63) / 64);
-        return (d.accum.chunk <= 8 && tiles_128x64 >= 1024) ? kMfma128x64 : kMfma64;
+        return (d.accum.chunk >= 16 && tiles_128x64 >= 1024) ? kMfma128x64 : kMfma64;
     }
     if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 1024) return 3;
     return 4;
@@ -205,6 +206,8 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
     g->C = d->beta != 0.0f ? d->C : nullptr;
     g->ldc = d->ldc;
     g->chunk = ac.chunk;
+    g->vec = (((uintptr_t)d->A | (uintptr_t)d->B) & 15u) == 0 && (d->lda & 3) == 0 &&
+             (d->ldb & 3) == 0;
     set_fast_path(g, ac.spec.mode == RNE && m <= 19, m);
     return LOF_OK;
 }

@@ -62,6 +62,7 @@ struct GemmArgs {
     int32_t elo_b;       // smallest normal exponent of the format, FP32-biased
     int32_t elo_d;       // the same, FP64-biased
     int32_t subn;        // the float fast path covers part of the format's subnormal range
+    int32_t vec;         // A, B 16-byte aligned with lda, ldb multiples of 4 (float4 loads)
     int32_t mfma_mode;   // MFMA tile: 0 exactness test, 1 every chunk on the VALU chain,
                          // 2 every chunk on MFMA (testing only: skips the test)
 };

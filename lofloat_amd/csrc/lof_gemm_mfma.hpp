@@ -7,7 +7,8 @@
 //
 // Exactness test, per workgroup tile and 32-wide K slice (every chunk of the slice is a
 // subset of it): with amax_A, amax_B the largest magnitudes in the A and B slice tiles and
-// lsb_A, lsb_B the smallest exponents of a set significand bit among their nonzero values,
+// lsb_A, lsb_B lower bounds on the exponents of the lowest set significand bits of their
+// nonzero values (smallest exponent plus smallest trailing-zero count of the significands),
 // every product is a multiple of 2^L, L = lsb_A + lsb_B, and every partial sum is bounded by
 // c * amax_A * amax_B < 2^(log2 c + e_A + e_B + 2).  The slice is exact when
 //     L >= -126                      (no FP32-subnormal product or sum; no subnormal input)
@@ -22,8 +23,9 @@
 // D[8 * (r / 4) + 4 * (l / 32) + r % 4][l % 32].
 //
 // Pipelining: the next K slice is loaded from HBM into registers while the current slice
-// computes from LDS; the chunk length is a template parameter so a chunk's LDS reads are
-// issued together and its MFMAs run back to back.
+// computes from one LDS buffer, then stored to the other (one barrier per slice); interior
+// slices use float4 loads.  The chunk length is a template parameter so a chunk's LDS reads
+// are issued together and its MFMAs run back to back.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,23 +36,19 @@ namespace lofamd {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// Smallest exponent of a set significand bit of a nonzero float; FP32-subnormal values
-// report -1000 so that a slice holding one never takes the MFMA path.
-__device__ __forceinline__ int lsb_exp(uint32_t absbits) {
-    const uint32_t be = absbits >> 23;
-    if (be == 0u) return -1000;
-    const uint32_t sig = (absbits & 0x7FFFFFu) | 0x800000u;
-    return (int)be - 150 + __builtin_ctz(sig);
-}
-
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
     return v;
 }
-__device__ __forceinline__ int wave_min_i32(int v) {
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= (uint32_t)__shfl_xor((int)v, o, 64);
     return v;
 }
 
@@ -120,28 +118,56 @@ struct MfmaTile {
     static constexpr int BK = 32;
     static constexpr int LDA_S = BM + 4;   // As[k][m], rows padded to keep float4 alignment
     static constexpr int LDB_S = BN + 4;   // Bs[k][n]
-    static constexpr int SMEM_FLOATS = BK * LDA_S + BK * LDB_S + 16;
-    static constexpr int A_PER_THREAD = BM * BK / 256;
-    static constexpr int B_PER_THREAD = BN * BK / 256;
+    static constexpr int BUF_FLOATS = BK * LDA_S + BK * LDB_S;
+    static constexpr int STAT_WORDS = 4 * 6;   // [wave][amaxA, minA, orA, amaxB, minB, orB]
+    static constexpr int SMEM_FLOATS = 2 * (BUF_FLOATS + STAT_WORDS);
+    static constexpr int NA4 = BM * BK / 1024;   // float4 of the A slice per thread
+    static constexpr int NB4 = BN * BK / 1024;
 };
+
+// Exactness statistics of a slice operand, accumulated per element v (bit pattern):
+//   amax = max(|v| << 1)           the largest magnitude (sign shifted out)
+//   mnk  = min((|v| << 1) - 1)     the smallest nonzero magnitude (zeros wrap to ~0)
+//   orb  = OR of the patterns      its low 23 bits give the smallest trailing-zero count
+// so every nonzero value's lowest set bit is at exponent >= e_min - 150 + ctz(orb | 2^23).
+struct SliceStat {
+    uint32_t amax = 0u, mnk = 0xFFFFFFFFu, orb = 0u;
+    __device__ __forceinline__ void add(float v) {
+        const uint32_t b = __float_as_uint(v);
+        const uint32_t sh = b << 1;
+        amax = max(amax, sh);
+        mnk = min(mnk, sh - 1u);
+        orb |= b;
+    }
+    __device__ __forceinline__ void add(const float4& v) { add(v.x); add(v.y); add(v.z); add(v.w); }
+};
+
+// lowest-set-bit exponent bound and largest exponent of an operand; false if the operand
+// holds an FP32-subnormal value (excluded from the MFMA path)
+__device__ __forceinline__ bool slice_bounds(uint32_t amax, uint32_t mnk, uint32_t orb, int& lsb,
+                                             int& emax) {
+    const uint32_t mn = (mnk + 1u) >> 1;   // smallest nonzero |v| bits
+    const int emin_b = (int)(mn >> 23);
+    lsb = emin_b - 150 + __builtin_ctz((orb & 0x7FFFFFu) | 0x800000u);
+    emax = (int)(amax >> 24) - 127;
+    return emin_b != 0;
+}
 
 template <int WMB, int WNB, int CHUNK, bool SUBN>
 __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, int tile_n,
                                                int tid, float* smem) {
     using T = MfmaTile<WMB, WNB>;
     constexpr int BM = T::BM, BN = T::BN, BK = T::BK, LDA_S = T::LDA_S, LDB_S = T::LDB_S;
-    constexpr int NA = T::A_PER_THREAD, NB = T::B_PER_THREAD;
+    constexpr int NA4 = T::NA4, NB4 = T::NB4;
     constexpr int LOG2C = CHUNK == 2 ? 1 : CHUNK == 4 ? 2 : CHUNK == 8 ? 3 : CHUNK == 16 ? 4 : 5;
     static_assert(CHUNK >= 2 && CHUNK <= BK && (BK % CHUNK) == 0, "chunk must divide 32");
-    float* As = smem;
-    float* Bs = smem + BK * LDA_S;
-    uint32_t* stat = (uint32_t*)(Bs + BK * LDB_S);   // [4 waves][amaxA, lsbA, amaxB, lsbB]
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int l32 = lane & 31, lhi = lane >> 5;
     const int m0 = tile_m * BM, n0 = tile_n * BN;
     const int am0 = wm * 32 * WMB, bn0 = wn * 32 * WNB;   // this wave's sub-tile
+    const bool interior_mn = m0 + BM <= g.M && n0 + BN <= g.N;
 
     float acc[WMB][WNB][16];
     f32x16 part[WMB][WNB];
@@ -152,92 +178,148 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
 
-    // global -> registers for the slice at k0, zero outside the matrix.  The loads are
-    // unconditional from clamped (always valid) addresses, so they issue without branches;
-    // M, N, K >= 1 whenever a tile runs.
-    float va[NA], vb[NB];
+    // Slice element mapping: float4 f = tid + it * 256 of the A slice holds
+    //   ta = 0 (A row-major, k contiguous): row m = f / 8, k = 4 * (f % 8) .. + 3
+    //   ta = 1 (A stored K x M, m contiguous): k = f / (BM / 4), m = 4 * (f % (BM / 4)) .. + 3
+    // and likewise for B (tb = 0: n contiguous; tb = 1: k contiguous).
+    float4 va[NA4], vb[NB4];
+    auto a_coord = [&](int it, int j, int& m, int& k) {
+        const int f = tid + it * 256;
+        if (g.ta) { k = f / (BM / 4); m = 4 * (f % (BM / 4)) + j; }
+        else { m = f / 8; k = 4 * (f % 8) + j; }
+    };
+    auto b_coord = [&](int it, int j, int& n, int& k) {
+        const int f = tid + it * 256;
+        if (g.tb) { n = f / 8; k = 4 * (f % 8) + j; }
+        else { k = f / (BN / 4); n = 4 * (f % (BN / 4)) + j; }
+    };
     auto load_slice = [&](int k0) {
+        if (g.vec && interior_mn && k0 + BK <= g.K) {   // whole slice inside: float4 loads
 #pragma unroll
-        for (int it = 0; it < NA; ++it) {
-            const int idx = tid + it * 256;
-            int m, k;
-            if (g.ta) { m = idx % BM; k = idx / BM; }   // A stored K x M: contiguous in m
-            else { k = idx % BK; m = idx / BK; }        // A row-major: contiguous in k
-            const int gm = m0 + m, gk = k0 + k;
-            const int cm = min(gm, g.M - 1), ck = min(gk, g.K - 1);
-            const float v = g.ta ? g.A[(int64_t)ck * g.lda + cm] : g.A[(int64_t)cm * g.lda + ck];
-            va[it] = (gm < g.M && gk < g.K) ? v : 0.0f;
-        }
+            for (int it = 0; it < NA4; ++it) {
+                int m, k;
+                a_coord(it, 0, m, k);
+                const float* p = g.ta ? g.A + (int64_t)(k0 + k) * g.lda + (m0 + m)
+                                      : g.A + (int64_t)(m0 + m) * g.lda + (k0 + k);
+                va[it] = *(const float4*)p;
+            }
 #pragma unroll
-        for (int it = 0; it < NB; ++it) {
-            const int idx = tid + it * 256;
-            int n, k;
-            if (g.tb) { k = idx % BK; n = idx / BK; }   // B stored N x K: contiguous in k
-            else { n = idx % BN; k = idx / BN; }        // B row-major: contiguous in n
-            const int gn = n0 + n, gk = k0 + k;
-            const int cn = min(gn, g.N - 1), ck = min(gk, g.K - 1);
-            const float v = g.tb ? g.B[(int64_t)cn * g.ldb + ck] : g.B[(int64_t)ck * g.ldb + cn];
-            vb[it] = (gn < g.N && gk < g.K) ? v : 0.0f;
+            for (int it = 0; it < NB4; ++it) {
+                int n, k;
+                b_coord(it, 0, n, k);
+                const float* p = g.tb ? g.B + (int64_t)(n0 + n) * g.ldb + (k0 + k)
+                                      : g.B + (int64_t)(k0 + k) * g.ldb + (n0 + n);
+                vb[it] = *(const float4*)p;
+            }
+        } else {   // edge: element loads from clamped addresses, zero outside the matrix
+#pragma unroll
+            for (int it = 0; it < NA4; ++it) {
+                float e[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    int m, k;
+                    a_coord(it, j, m, k);
+                    const int gm = m0 + m, gk = k0 + k;
+                    const int cm = min(gm, g.M - 1), ck = min(gk, g.K - 1);
+                    const float v = g.ta ? g.A[(int64_t)ck * g.lda + cm] : g.A[(int64_t)cm * g.lda + ck];
+                    e[j] = (gm < g.M && gk < g.K) ? v : 0.0f;
+                }
+                va[it] = make_float4(e[0], e[1], e[2], e[3]);
+            }
+#pragma unroll
+            for (int it = 0; it < NB4; ++it) {
+                float e[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    int n, k;
+                    b_coord(it, j, n, k);
+                    const int gn = n0 + n, gk = k0 + k;
+                    const int cn = min(gn, g.N - 1), ck = min(gk, g.K - 1);
+                    const float v = g.tb ? g.B[(int64_t)cn * g.ldb + ck] : g.B[(int64_t)ck * g.ldb + cn];
+                    e[j] = (gn < g.N && gk < g.K) ? v : 0.0f;
+                }
+                vb[it] = make_float4(e[0], e[1], e[2], e[3]);
+            }
         }
     };
-    // registers -> LDS, with this slice's exactness statistics written to `stat`
-    auto store_slice = [&]() {
-        uint32_t amaxA = 0u, amaxB = 0u;
-        int lsbA = 1000, lsbB = 1000;
+    // registers -> LDS buffer `buf`, with the slice's exactness statistics
+    auto store_slice = [&](int buf) {
+        float* As = smem + buf * (T::BUF_FLOATS + T::STAT_WORDS);
+        float* Bs = As + BK * LDA_S;
+        uint32_t* stat = (uint32_t*)(Bs + BK * LDB_S);
+        SliceStat sa, sb;
 #pragma unroll
-        for (int it = 0; it < NA; ++it) {
-            const int idx = tid + it * 256;
+        for (int it = 0; it < NA4; ++it) {
             int m, k;
-            if (g.ta) { m = idx % BM; k = idx / BM; }
-            else { k = idx % BK; m = idx / BK; }
-            As[k * LDA_S + m] = va[it];
-            const uint32_t ab = __float_as_uint(va[it]) & 0x7FFFFFFFu;
-            amaxA = max(amaxA, ab);
-            if (ab) lsbA = min(lsbA, lsb_exp(ab));
+            a_coord(it, 0, m, k);
+            if (g.ta) {
+                *(float4*)&As[k * LDA_S + m] = va[it];
+            } else {
+                As[(k + 0) * LDA_S + m] = va[it].x;
+                As[(k + 1) * LDA_S + m] = va[it].y;
+                As[(k + 2) * LDA_S + m] = va[it].z;
+                As[(k + 3) * LDA_S + m] = va[it].w;
+            }
+            sa.add(va[it]);
         }
 #pragma unroll
-        for (int it = 0; it < NB; ++it) {
-            const int idx = tid + it * 256;
+        for (int it = 0; it < NB4; ++it) {
             int n, k;
-            if (g.tb) { k = idx % BK; n = idx / BK; }
-            else { n = idx % BN; k = idx / BN; }
-            Bs[k * LDB_S + n] = vb[it];
-            const uint32_t ab = __float_as_uint(vb[it]) & 0x7FFFFFFFu;
-            amaxB = max(amaxB, ab);
-            if (ab) lsbB = min(lsbB, lsb_exp(ab));
+            b_coord(it, 0, n, k);
+            if (g.tb) {
+                Bs[(k + 0) * LDB_S + n] = vb[it].x;
+                Bs[(k + 1) * LDB_S + n] = vb[it].y;
+                Bs[(k + 2) * LDB_S + n] = vb[it].z;
+                Bs[(k + 3) * LDB_S + n] = vb[it].w;
+            } else {
+                *(float4*)&Bs[k * LDB_S + n] = vb[it];
+            }
+            sb.add(vb[it]);
         }
-        amaxA = wave_max_u32(amaxA);
-        amaxB = wave_max_u32(amaxB);
-        lsbA = wave_min_i32(lsbA);
-        lsbB = wave_min_i32(lsbB);
+        sa.amax = wave_max_u32(sa.amax);
+        sb.amax = wave_max_u32(sb.amax);
+        sa.mnk = wave_min_u32(sa.mnk);
+        sb.mnk = wave_min_u32(sb.mnk);
+        sa.orb = wave_or_u32(sa.orb);
+        sb.orb = wave_or_u32(sb.orb);
         if (lane == 0) {
-            stat[wave * 4 + 0] = amaxA;
-            stat[wave * 4 + 1] = (uint32_t)lsbA;
-            stat[wave * 4 + 2] = amaxB;
-            stat[wave * 4 + 3] = (uint32_t)lsbB;
+            uint32_t* st = stat + wave * 6;
+            st[0] = sa.amax; st[1] = sa.mnk; st[2] = sa.orb;
+            st[3] = sb.amax; st[4] = sb.mnk; st[5] = sb.orb;
         }
     };
 
     if (g.K > 0) {
         load_slice(0);
-        store_slice();
+        store_slice(0);
     }
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        __syncthreads();   // slice k0 and its statistics are in LDS
+    int buf = 0;
+    for (int k0 = 0; k0 < g.K; k0 += BK, buf ^= 1) {
+        // Slice k0 and its statistics are in LDS buffer `buf`; every wave is done with the
+        // other buffer (slice k0 - 32), which the store below refills.
+        __syncthreads();
+        const float* As = smem + buf * (T::BUF_FLOATS + T::STAT_WORDS);
+        const float* Bs = As + BK * LDA_S;
+        const uint32_t* stat = (const uint32_t*)(Bs + BK * LDB_S);
         bool exact;
         {
-            const uint32_t amaxA = max(max(stat[0], stat[4]), max(stat[8], stat[12]));
-            const uint32_t amaxB = max(max(stat[2], stat[6]), max(stat[10], stat[14]));
-            const int lsbA = min(min((int)stat[1], (int)stat[5]), min((int)stat[9], (int)stat[13]));
-            const int lsbB = min(min((int)stat[3], (int)stat[7]), min((int)stat[11], (int)stat[15]));
-            if (amaxA == 0u || amaxB == 0u) {
+            uint32_t amA = 0u, mnA = 0xFFFFFFFFu, orA = 0u, amB = 0u, mnB = 0xFFFFFFFFu, orB = 0u;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                amA = max(amA, stat[w * 6 + 0]); mnA = min(mnA, stat[w * 6 + 1]); orA |= stat[w * 6 + 2];
+                amB = max(amB, stat[w * 6 + 3]); mnB = min(mnB, stat[w * 6 + 4]); orB |= stat[w * 6 + 5];
+            }
+            if (amA == 0u || amB == 0u) {
                 exact = true;                              // every product is zero
-            } else if (amaxA >= 0x7F800000u || amaxB >= 0x7F800000u) {
+            } else if (amA >= 0xFF000000u || amB >= 0xFF000000u) {
                 exact = false;                             // inf or NaN in the slice
             } else {
+                int lsbA, lsbB, eA, eB;
+                const bool normA = slice_bounds(amA, mnA, orA, lsbA, eA);
+                const bool normB = slice_bounds(amB, mnB, orB, lsbB, eB);
                 const int L = lsbA + lsbB;
-                const int top = LOG2C + (int)(amaxA >> 23) - 127 + (int)(amaxB >> 23) - 127 + 2;
-                exact = L >= -126 && top <= L + 24 && top <= 127;
+                const int top = LOG2C + eA + eB + 2;
+                exact = normA && normB && L >= -126 && top <= L + 24 && top <= 127;
             }
             if (g.mfma_mode != 0) exact = g.mfma_mode == 2;
         }
@@ -320,8 +402,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
 #pragma unroll
                 for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
         }
-        __syncthreads();   // every wave is done reading slice k0
-        if (has_next) store_slice();
+        if (has_next) store_slice(buf ^ 1);
     }
 
     // epilogue (cutlass LinearCombination): D = alpha * acc, or fma(alpha, acc, beta * C)
