@@ -160,7 +160,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(),
-                     "kernel": "lofamd::k_mx<RNE,0,2>", "algorithmic_bytes": int(bytes_per)},
+                     "kernel": "lofamd::k_mx<0, 0, 8, true>", "algorithmic_bytes": int(bytes_per)},
         "cpu_baseline": {"value": cpu_gbs, "unit": "GB/s", "cores": cores,
                          "kind": "port",
                          "sample": f"full 8192x8192 matrix, best of {reps} runs of the C oracle "
