@@ -9,7 +9,8 @@ Also reported (``extra``): elementwise virtual_round of 2^24 / 2^28 values to (8
 with e2m1 and with SR-A random words; the emulated-accumulation GEMM at M=N=K=4096
 (TFLOP/s = 2MNK / t) for 10-bit mantissa and 5e10m accumulators at chunks 1/8/32; one
 LoF_Linear forward+backward step (X 8192x4096, W 4096x4096); and the grouped launch of 64
-uneven rounding jobs + 12 GEMMs against per-job launches, both captured in HIP graphs.
+uneven rounding jobs + 12 GEMMs (e4m3 operands) against per-job launches, both captured in
+HIP graphs.
 
 The CPU baseline is the C oracle (a restatement of the reference's scalar semantics) run
 with OpenMP on the host cores on the same workload.
@@ -73,9 +74,10 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run N device copies of the 8192x8192 matrix (a plain copy "
                          "kernel of known traffic, to calibrate HBM byte counters)")
-    ap.add_argument("--only", choices=["gemm"], default=None,
+    ap.add_argument("--only", choices=["gemm", "linear", "grouped"], default=None,
                     help="run one workload alone (for profiling): 'gemm' = the 4096^3 "
-                         "emulated-accumulation GEMM with --chunk / --accum / --inputs")
+                         "emulated-accumulation GEMM with --chunk / --accum / --inputs; "
+                         "'linear' = the LoF_Linear step; 'grouped' = the batched launch")
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--accum", choices=["mant10", "5e10m"], default="5e10m")
     ap.add_argument("--inputs", choices=["e4m3", "8e7m"], default="e4m3")
@@ -98,6 +100,12 @@ def main():
     e8m0 = lof.create_e8m0_params()
     if args.only == "gemm":
         gemm_only(lof, torch, dev, args)
+        return
+    if args.only in ("linear", "grouped"):
+        import numpy as np
+        out = {}
+        (linear_steps if args.only == "linear" else grouped_vs_separate)(lof, torch, np, dev, out)
+        print(json.dumps(out), flush=True)
         return
 
     # ---------------------------------------------------------------- primary: MX 8192^2
@@ -243,7 +251,16 @@ def extras(lof, torch, np, oracle, dev):
         del A, B
     out["gemm_roofline_peak_TFLOPs"] = VALU_FP32_PEAK_TFLOPS
 
-    # C4: LoF_Linear forward + backward step
+    linear_steps(lof, torch, np, dev, out)
+    grouped_vs_separate(lof, torch, np, dev, out)
+    return out
+
+
+def linear_steps(lof, torch, np, dev, out):
+    """LoF_Linear forward + backward: X 8192x4096, W 4096x4096, MX-e4m3 operands, e5m2 SR-A
+    gradients, accumulators 5e10m chunk 32 and 10-bit mantissa chunk 1."""
+    e4m3 = lof.make_format(8, 3, 7)
+    fp16 = lof.make_format(16, 10, 15)
     M, K, Nout = 8192, 4096, 4096
     for label, kw in (("5e10m_c32", dict(accum_format=fp16, accum_chunk=32)),
                       ("mant10_c1", dict(accum_mant_bits=10, accum_chunk=1))):
@@ -267,7 +284,13 @@ def extras(lof, torch, np, oracle, dev):
         del lin, X, G
     torch.cuda.empty_cache()
 
-    # C5: grouped launch vs per-job launches (both HIP-graph captured)
+
+def grouped_vs_separate(lof, torch, np, dev, out):
+    """64 uneven rounding jobs (own format and mode each) + 12 emulated GEMMs of MLP shapes,
+    one persistent launch vs per-job launches, both captured in HIP graphs.  GEMM operands
+    are pre-rounded to e4m3, accumulator 5e10m with chunk 32."""
+    fp16 = lof.make_format(16, 10, 15)
+    e4m3 = lof.make_format(8, 3, 7)
     rng = np.random.default_rng(5)
     jobs = []
     modes = [0, 12, 4, 1, 3]
@@ -284,8 +307,8 @@ def extras(lof, torch, np, oracle, dev):
     gemm_flop = 0.0
     for j in range(12):
         Mg, Ng, Kg = (int(rng.choice(dims)) for _ in range(3))
-        A = torch.randn(Mg, Kg, device=dev) / 64
-        B = torch.randn(Kg, Ng, device=dev) / 64
+        A = lof.virtual_round(torch.randn(Mg, Kg, device=dev) / 64, e4m3)
+        B = lof.virtual_round(torch.randn(Kg, Ng, device=dev) / 64, e4m3)
         jobs.append(lof.GemmJob(A, B, lof.AccumSpec(fmt=fp16, chunk=32)))
         gemm_flop += 2.0 * Mg * Ng * Kg
     cfg, tune_ms = lof.grouped.autotune(jobs, iters=1)
@@ -302,7 +325,6 @@ def extras(lof, torch, np, oracle, dev):
                 "grouped_round_elems": int(sum(j.input.numel() for j in jobs[:64])),
                 "grouped_gemm_TFLOP": round(gemm_flop / 1e12, 3)})
     log(f"grouped {ms_g:.1f} ms vs separate {ms_s:.1f} ms, bit-exact {same}")
-    return out
 
 
 if __name__ == "__main__":

@@ -9,7 +9,6 @@
 #include <tuple>
 
 #include "lof_gemm.hpp"
-#include "lof_gemm_mfma.hpp"
 #include "lof_host.hpp"
 
 namespace lofamd {
@@ -226,6 +225,7 @@ lof_status launch_gemm(const GemmArgs& g, int config, hipStream_t st) {
 }
 
 int gemm_num_configs() { return kNumConfigs; }
+bool gemm_chunk_fits_mfma(int chunk) { return chunk_fits_mfma(chunk); }
 int gemm_config_bm(int c) { return kConfigs[c].bm; }
 int gemm_config_bn(int c) { return kConfigs[c].bn; }
 

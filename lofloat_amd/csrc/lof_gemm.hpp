@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "lof_gemm_kernels.hpp"
+#include "lof_gemm_mfma.hpp"
 #include "lofloat.h"
 
 namespace lofamd {
@@ -15,5 +16,7 @@ int resolve_gemm_config(const lof_gemm_desc& d, int config);
 int gemm_num_configs();
 int gemm_config_bm(int c);
 int gemm_config_bn(int c);
+// the chunk lengths whose sums can run on MFMA (2, 4, 8, 16, 32)
+bool gemm_chunk_fits_mfma(int chunk);
 
 }  // namespace lofamd

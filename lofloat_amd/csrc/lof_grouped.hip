@@ -1,7 +1,9 @@
 // lof_grouped.hip -- one persistent launch over many rounding / MX / GEMM jobs.
 //
 // The plan uploads one GroupTask per job and the first tile index of each job (jobs are
-// ordered GEMMs first, largest first, then rounding jobs, largest first).  A fixed grid of
+// ordered GEMMs first, largest first, then rounding jobs, largest first).  GEMMs whose
+// k-chunk fits MFMA use the 64 x 64 MFMA chunk-sum tile, the others the config's VALU tile.
+// A fixed grid of
 // workgroups (as many as fit on the device at once) repeatedly claims the next tile with
 // an atomic counter, finds its job by binary search over the start indices, and runs the
 // job's tile routine: the same device routines as the single-job kernels, dispatched once
@@ -38,6 +40,7 @@ struct GroupTask {
     int32_t mode;     // element rounding mode
     int32_t rsrc;     // 0 no random words, 1 bits array, 2 generated words
     int32_t tiles_n;  // GEMM: tiles along N
+    int32_t mfma;     // GEMM: 64 x 64 MFMA chunk-sum tiles (else the config's VALU tile)
     RoundArgs r;
     MxArgs x;
     GemmArgs g;
@@ -103,11 +106,34 @@ __device__ __noinline__ void grouped_mx(const MxArgs& a, int mode, int rsrc, int
     }
 }
 
+// GEMM tiles whose chunk sums run on MFMA: the 64 x 64 tile per (chunk, subnormal fast
+// range) variant, inlined (as non-inlined calls they raise the kernel's register count
+// enough to halve its occupancy).
+template <int CHUNK, bool SUBN>
+__device__ __forceinline__ void g_mfma_tile(const GemmArgs& g, int tm, int tn, int tid,
+                                         float* smem) {
+    gemm_tile_mfma<1, 1, CHUNK, SUBN>(g, tm, tn, tid, smem);
+}
+
+__device__ __forceinline__ void grouped_mfma(const GemmArgs& g, int tm, int tn, int tid,
+                                          float* smem) {
+#define LOF_GM(C)                                                   \
+    if (g.chunk == C) {                                             \
+        if (g.subn) g_mfma_tile<C, true>(g, tm, tn, tid, smem);     \
+        else g_mfma_tile<C, false>(g, tm, tn, tid, smem);           \
+        return;                                                     \
+    }
+    LOF_GM(2) LOF_GM(4) LOF_GM(8) LOF_GM(16) LOF_GM(32)
+#undef LOF_GM
+}
+
 template <class C>
 __global__ __launch_bounds__(256) void k_grouped(const GroupTask* __restrict__ tasks,
                                                  const int64_t* __restrict__ starts, int ntasks,
                                                  int64_t total, unsigned long long* counters) {
-    __shared__ float smem[gemm_smem_floats<C::bm, C::bn, C::bk>()];
+    constexpr int kValuSmem = gemm_smem_floats<C::bm, C::bn, C::bk>();
+    constexpr int kMfmaSmem = MfmaTile<1, 1>::SMEM_FLOATS;
+    __shared__ __align__(16) float smem[kValuSmem > kMfmaSmem ? kValuSmem : kMfmaSmem];
     __shared__ unsigned long long s_tile;
     const int tid = threadIdx.x;
     for (;;) {
@@ -134,7 +160,9 @@ __global__ __launch_bounds__(256) void k_grouped(const GroupTask* __restrict__ t
             grouped_mx(T.x, T.mode, T.rsrc, local, C::sub, tid);
             break;
         default:
-            if (T.g.subn)
+            if (T.mfma)
+                grouped_mfma(T.g, (int)(local / T.tiles_n), (int)(local % T.tiles_n), tid, smem);
+            else if (T.g.subn)
                 gemm_tile<C::bm, C::bn, C::tm, C::tn, C::bk, true>(
                     T.g, (int)(local / T.tiles_n), (int)(local % T.tiles_n), tid, smem);
             else
@@ -227,8 +255,13 @@ lof_status build_task(const lof_group_item& it, const GroupedConfig& cfg, GroupT
     case LOF_ITEM_GEMM:
         s = prepare_gemm(&it.gemm, &t->g);
         if (s != LOF_OK) return s;
-        t->tiles_n = (it.gemm.N + cfg.bn - 1) / cfg.bn;
-        *tiles = (int64_t)t->tiles_n * ((it.gemm.M + cfg.bm - 1) / cfg.bm);
+        t->mfma = gemm_chunk_fits_mfma(it.gemm.accum.chunk) ? 1 : 0;
+        {
+            const int bm = t->mfma ? MfmaTile<1, 1>::BM : cfg.bm;
+            const int bn = t->mfma ? MfmaTile<1, 1>::BN : cfg.bn;
+            t->tiles_n = (it.gemm.N + bn - 1) / bn;
+            *tiles = (int64_t)t->tiles_n * ((it.gemm.M + bm - 1) / bm);
+        }
         if (it.gemm.M == 0 || it.gemm.N == 0) *tiles = 0;
         return LOF_OK;
     default:
