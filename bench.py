@@ -272,6 +272,15 @@ def linear_steps(lof, torch, np, dev, out):
             lin.weight.normal_(0, 0.02)
         X = (torch.randn(M, K, device=dev) * 0.02).requires_grad_(True)
         G = torch.randn(M, Nout, device=dev)
+        # the library's GEMM autotuner, on operands of the three products' shapes, layouts
+        # and formats (forward X_q W_q^T, dgrad G_q W_q, wgrad G_q^T X_q); LoF_Linear then
+        # runs with the tuned tiles
+        with torch.no_grad():
+            xq, wq, _ = lin.quantized_operands(X.detach())
+            gq = lof.virtual_round(G, lof.make_format(8, 2, 15), 3, 8, seed=9)
+            for A_, B_ in ((xq, wq.t()), (gq, wq), (gq.t(), xq)):
+                lof.gemm_autotune(A_, B_, lin.accum_spec(), iters=1)
+            del xq, wq, gq
 
         def step(i):
             X.grad = None

@@ -21,7 +21,7 @@ def _fmt(t):
                                      lof.formats.P3109_NaNChecker(t[0]))
 
 
-def _make_jobs(lof, dev, seed=0, n_round=24, n_gemm=3, max_log2=18):
+def _make_jobs(lof, dev, seed=0, n_round=24, n_gemm=4, max_log2=18):
     g = np.random.default_rng(seed)
     jobs, refs = [], []
     modes = [0, 12, 4, 1, 3]
@@ -55,7 +55,10 @@ def _make_jobs(lof, dev, seed=0, n_round=24, n_gemm=3, max_log2=18):
         M, N, K = (int(v) for v in g.choice([64, 96, 128, 200], 3))
         A = (g.standard_normal((M, K)) / 8).astype(np.float32)
         B = (g.standard_normal((K, N)) / 8).astype(np.float32)
-        chunk = [1, 8, 32][j % 3]
+        if j % 2 == 1:  # e4m3 operands: chunk sums exact, MFMA tiles take them
+            A = oracle.virtual_round(A / 8, (8, 3, 7, 1, 0, 0)).reshape(M, K)
+            B = oracle.virtual_round(B / 8, (8, 3, 7, 1, 0, 0)).reshape(K, N)
+        chunk = [1, 8, 32, 4][j % 4]
         acc = lof.AccumSpec(mant_bits=10, chunk=chunk) if j % 2 == 0 else \
             lof.AccumSpec(fmt=_fmt((16, 10, 15, 1, 0, 0)), chunk=chunk)
         oacc = oracle.accum_mant(10, chunk) if j % 2 == 0 else \
@@ -72,7 +75,7 @@ def _check(jobs, refs, what):
         assert_bits(j.out.cpu().numpy().ravel(), np.asarray(r).ravel(), f"{what} job {k}")
 
 
-@pytest.mark.parametrize("config", [0, 1, 2, 3])
+@pytest.mark.parametrize("config", [0, 1, 2, 3, 4, 5])
 def test_grouped_matches_oracle(dev, config):
     import lofloat_amd as lof
     jobs, refs = _make_jobs(lof, dev, seed=config)
