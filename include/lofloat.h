@@ -184,7 +184,8 @@ LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_s
 
 LOF_API int32_t lof_gemm_num_configs(void);
 /* 1 if tile config `config` can run a GEMM with k-chunk `chunk` (MFMA configs need chunk
- * in {2, 4, 8, 16, 32}), else 0.  No reference counterpart: the reference has one kernel. */
+ * in {1, 2, 4, 8, 16, 32}; chunk 1 runs on VALU), else 0.  No reference counterpart: the
+ * reference has one kernel. */
 LOF_API int32_t lof_gemm_config_valid(int32_t config, int32_t chunk);
 /* Writes a short "BMxBN/TMxTN" style name; returns LOF_ERR_INVALID_ARGUMENT if out of range. */
 LOF_API lof_status lof_gemm_config_name(int32_t config, char* buf, int32_t buflen);

@@ -35,7 +35,7 @@ struct GemmConfig {
     const char* name;
     int bm, bn;
     GemmKernel kernel[2];          // VALU tile, any chunk: [subn]
-    GemmKernel mfma[5][2];         // MFMA tile: [log2(chunk) - 1][subn], chunk in {2..32}
+    GemmKernel mfma[6][2];         // wave tile: [log2(chunk)][subn], chunk in {1, 2, ..., 32}
 };
 
 #define LOF_GEMM_CFG(BM, BN, TM, TN, BK)                                                    \
@@ -44,8 +44,8 @@ struct GemmConfig {
 #define LOF_MFMA_PAIR(WMB, WNB, C) {k_gemm_mfma<WMB, WNB, C, false>, k_gemm_mfma<WMB, WNB, C, true>}
 #define LOF_GEMM_MFMA_CFG(WMB, WNB)                                                         \
     {"mfma" #WMB #WNB, 64 * WMB, 64 * WNB, {nullptr, nullptr},                               \
-     {LOF_MFMA_PAIR(WMB, WNB, 2), LOF_MFMA_PAIR(WMB, WNB, 4), LOF_MFMA_PAIR(WMB, WNB, 8),     \
-      LOF_MFMA_PAIR(WMB, WNB, 16), LOF_MFMA_PAIR(WMB, WNB, 32)}}
+     {LOF_MFMA_PAIR(WMB, WNB, 1), LOF_MFMA_PAIR(WMB, WNB, 2), LOF_MFMA_PAIR(WMB, WNB, 4),     \
+      LOF_MFMA_PAIR(WMB, WNB, 8), LOF_MFMA_PAIR(WMB, WNB, 16), LOF_MFMA_PAIR(WMB, WNB, 32)}}
 
 const GemmConfig kConfigs[] = {
     LOF_GEMM_CFG(64, 64, 4, 4, 16),
@@ -60,18 +60,22 @@ const GemmConfig kConfigs[] = {
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kMfma128x64 = 6, kMfma64 = 7;
 
+// chunks whose sums can run on MFMA
 bool chunk_fits_mfma(int chunk) { return chunk >= 2 && chunk <= 32 && (32 % chunk) == 0; }
+// chunks the wave tiles (MFMA configs) take: those, and chunk 1 on VALU
+bool chunk_fits_wave_tile(int chunk) { return chunk == 1 || chunk_fits_mfma(chunk); }
 
 bool is_mfma(int config) { return kConfigs[config].kernel[0] == nullptr; }
 
 bool config_valid(int config, int chunk) {
-    return config >= 0 && config < kNumConfigs && (!is_mfma(config) || chunk_fits_mfma(chunk));
+    return config >= 0 && config < kNumConfigs &&
+           (!is_mfma(config) || chunk_fits_wave_tile(chunk));
 }
 
 GemmKernel pick_kernel(const GemmConfig& c, const GemmArgs& g) {
     if (c.kernel[0]) return c.kernel[g.subn ? 1 : 0];
     const int l2 = 31 - __builtin_clz((unsigned)g.chunk);
-    return c.mfma[l2 - 1][g.subn ? 1 : 0];
+    return c.mfma[l2][g.subn ? 1 : 0];
 }
 
 // autotuned choices, keyed by the problem's shape class
@@ -91,12 +95,12 @@ TuneKey tune_key(const lof_gemm_desc& d) {
                    d.accum.chunk};
 }
 
-// Untuned choice, from measurements at M = N = K = 4096 (e4m3 inputs): MFMA tiles for
-// chunks they take (64x64 for c <= 8, where the leaves dominate; 128x64 for c >= 16 when
-// that still gives >= 1024 tiles), the 32x64 VALU tile otherwise; the 32x32 VALU tile when
-// there are too few tiles to fill the chip.
+// Untuned choice, from measurements at M = N = K = 4096 (e4m3 / 8e7m inputs): wave tiles
+// for the chunks they take (64x64 for c <= 8, where the leaves dominate, chunk 1 included;
+// 128x64 for c >= 16 when that still gives >= 1024 tiles), the 32x64 VALU tile otherwise;
+// the 32x32 VALU tile when there are too few tiles to fill the chip.
 int default_config(const lof_gemm_desc& d) {
-    if (chunk_fits_mfma(d.accum.chunk)) {
+    if (chunk_fits_wave_tile(d.accum.chunk)) {
         const int64_t tiles_128x64 = (int64_t)((d.M + 127) / 128) * ((d.N + 63) / 64);
         return (d.accum.chunk >= 16 && tiles_128x64 >= 1024) ? kMfma128x64 : kMfma64;
     }
@@ -215,7 +219,7 @@ lof_status launch_gemm(const GemmArgs& g, int config, hipStream_t st) {
     if (g.M == 0 || g.N == 0) return LOF_OK;
     if (!config_valid(config, g.chunk))
         return set_error(LOF_ERR_UNSUPPORTED,
-                         "lof_gemm_f32: config %d needs a chunk in {2, 4, 8, 16, 32} (got %d)",
+                         "lof_gemm_f32: config %d needs a chunk in {1, 2, 4, 8, 16, 32} (got %d)",
                          config, g.chunk);
     const GemmConfig& c = kConfigs[config];
     dim3 grid((unsigned)((g.N + c.bn - 1) / c.bn), (unsigned)((g.M + c.bm - 1) / c.bm));

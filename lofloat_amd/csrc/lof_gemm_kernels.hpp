@@ -146,10 +146,13 @@ __device__ __noinline__ double leaf_slow(double p, double acc, const GemmArgs& g
     return leaf_general(p, acc, s, e, g);
 }
 
-// One emulated accumulation step (fast path, else the general path).
+// One emulated accumulation step (fast path, else the general path).  The fast path's
+// domain reaches into the format's subnormal range when g.subn is set, so the rounding
+// position must follow it there.
 __device__ __forceinline__ double leaf(double p, double acc, const GemmArgs& g) {
     double r;
-    if (__builtin_expect(!leaf_fast<false>(p, acc, g, r), 0)) r = leaf_slow(p, acc, g);
+    const bool ok = g.subn ? leaf_fast<true>(p, acc, g, r) : leaf_fast<false>(p, acc, g, r);
+    if (__builtin_expect(!ok, 0)) r = leaf_slow(p, acc, g);
     return r;
 }
 

@@ -1,7 +1,9 @@
 // lof_gemm_mfma.hpp -- emulated-accumulation GEMM with MFMA chunk sums (gfx950).
 //
 // For chunk c > 1 the accumulation rule is: p = FP32 fmaf chain over the chunk's c products
-// (ascending k, starting at +0), then acc <- round(RTO(p + acc)).  When every partial sum of
+// (ascending k, starting at +0), then acc <- round(RTO(p + acc)).  The same tile also runs
+// chunk 1 (acc <- round(RTO_fma(a, b, acc)) per k) on VALU, with float products where a
+// slice's products are exact in FP32 (operand significand widths summing to <= 24).  When every partial sum of
 // the chunk is exactly representable in FP32, p is the exact sum of the products and any
 // summation order gives it; v_mfma_f32_32x32x2f32 then computes the chunk sums.
 //
@@ -91,9 +93,10 @@ __device__ __forceinline__ bool leaf_f_fast(float p, float acc, const GemmArgs& 
     return ((sb & 0x7FFFFFFFu) - g.lo_f) < g.range_f;
 }
 
-// Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast results are committed
-// where valid, lanes needing the general path are recorded in a bit mask, and one
-// wave-uniform branch runs the general path for them (acc still holds their old value).
+// Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
+// results are committed where valid, outputs needing the general path are recorded in a bit
+// mask, and one wave-uniform branch runs the general path for them (acc still holds their
+// old value).
 template <bool SUBN>
 __device__ __forceinline__ void leaves_f16(const f32x16& p, float* acc, const GemmArgs& g) {
     uint32_t slow = 0;
@@ -159,8 +162,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     using T = MfmaTile<WMB, WNB>;
     constexpr int BM = T::BM, BN = T::BN, BK = T::BK, LDA_S = T::LDA_S, LDB_S = T::LDB_S;
     constexpr int NA4 = T::NA4, NB4 = T::NB4;
-    constexpr int LOG2C = CHUNK == 2 ? 1 : CHUNK == 4 ? 2 : CHUNK == 8 ? 3 : CHUNK == 16 ? 4 : 5;
-    static_assert(CHUNK >= 2 && CHUNK <= BK && (BK % CHUNK) == 0, "chunk must divide 32");
+    constexpr int LOG2C = CHUNK == 1 ? 0 : CHUNK == 2 ? 1 : CHUNK == 4 ? 2 : CHUNK == 8 ? 3
+                        : CHUNK == 16 ? 4 : 5;
+    static_assert(CHUNK >= 1 && CHUNK <= BK && (BK % CHUNK) == 0, "chunk must divide 32");
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -318,8 +322,16 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 const bool normA = slice_bounds(amA, mnA, orA, lsbA, eA);
                 const bool normB = slice_bounds(amB, mnB, orB, lsbB, eB);
                 const int L = lsbA + lsbB;
-                const int top = LOG2C + eA + eB + 2;
-                exact = normA && normB && L >= -126 && top <= L + 24 && top <= 127;
+                if constexpr (CHUNK == 1) {
+                    // every product a*b exact in FP32: significand widths sum to <= 24, and
+                    // no product overflows or falls below 2^-149
+                    const int wA = 24 - __builtin_ctz((orA & 0x7FFFFFu) | 0x800000u);
+                    const int wB = 24 - __builtin_ctz((orB & 0x7FFFFFu) | 0x800000u);
+                    exact = normA && normB && wA + wB <= 24 && eA + eB + 2 <= 127 && L >= -149;
+                } else {
+                    const int top = LOG2C + eA + eB + 2;
+                    exact = normA && normB && L >= -126 && top <= L + 24 && top <= 127;
+                }
             }
             if (g.mfma_mode != 0) exact = g.mfma_mode == 2;
         }
@@ -327,6 +339,52 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         if (has_next) load_slice(k0 + BK);   // in flight while this slice computes
 
         const int kmax = min(BK, g.K - k0);
+        if constexpr (CHUNK == 1) {
+            // acc <- round(RTO_fma(a, b, acc)) for every k: float products where the slice's
+            // products are exact in FP32, double products otherwise
+            for (int kk = 0; kk < kmax; ++kk) {
+                float4 a4[WMB][4];
+                float b[WNB];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq)
+                        a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj) b[bj] = Bs[kk * LDB_S + bn0 + bj * 32 + l32];
+                if (exact) {
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj) {
+                            f32x16 pv;
+#pragma unroll
+                            for (int rq = 0; rq < 4; ++rq) {
+                                pv[rq * 4 + 0] = __fmul_rn(a4[bi][rq].x, b[bj]);
+                                pv[rq * 4 + 1] = __fmul_rn(a4[bi][rq].y, b[bj]);
+                                pv[rq * 4 + 2] = __fmul_rn(a4[bi][rq].z, b[bj]);
+                                pv[rq * 4 + 3] = __fmul_rn(a4[bi][rq].w, b[bj]);
+                            }
+                            leaves_f16<SUBN>(pv, acc[bi][bj], g);
+                        }
+                } else {   // products wider than FP32: exact in double, one leaf at a time
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj) {
+                            const double bd = (double)b[bj];
+                            const float av[16] = {a4[bi][0].x, a4[bi][0].y, a4[bi][0].z, a4[bi][0].w,
+                                                  a4[bi][1].x, a4[bi][1].y, a4[bi][1].z, a4[bi][1].w,
+                                                  a4[bi][2].x, a4[bi][2].y, a4[bi][2].z, a4[bi][2].w,
+                                                  a4[bi][3].x, a4[bi][3].y, a4[bi][3].z, a4[bi][3].w};
+#pragma unroll
+                            for (int o = 0; o < 16; ++o)
+                                acc[bi][bj][o] = (float)leaf((double)av[o] * bd,
+                                                             (double)acc[bi][bj][o], g);
+                        }
+                }
+            }
+        } else {
 #pragma unroll
         for (int q = 0; q < BK / CHUNK; ++q) {
             const int kb = q * CHUNK;
@@ -401,6 +459,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
                 for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
+        }
         }
         if (has_next) store_slice(buf ^ 1);
     }

@@ -1,9 +1,9 @@
 // lof_grouped.hip -- one persistent launch over many rounding / MX / GEMM jobs.
 //
 // The plan uploads one GroupTask per job and the first tile index of each job (jobs are
-// ordered GEMMs first, largest first, then rounding jobs, largest first).  GEMMs whose
-// k-chunk fits MFMA use the config's MFMA chunk-sum tile (64 x 64 or 128 x 64), the others
-// its VALU tile.
+// ordered GEMMs first, largest first, then rounding jobs, largest first).  GEMMs with
+// k-chunk 8, 16 or 32 use the 64 x 64 MFMA chunk-sum tile, the others the config's VALU
+// tile.
 // A fixed grid of
 // workgroups (as many as fit on the device at once) repeatedly claims the next tile with
 // an atomic counter, finds its job by binary search over the start indices, and runs the
@@ -108,8 +108,8 @@ __device__ __noinline__ void grouped_mx(const MxArgs& a, int mode, int rsrc, int
     }
 }
 
-// GEMM tiles whose chunk sums run on MFMA: the 64 x 64 tile per (chunk, subnormal fast
-// range) variant, inlined (as non-inlined calls they raise the kernel's register count
+// GEMM tiles whose chunk sums run on MFMA (chunks 8, 16, 32): the 64 x 64 tile per
+// (chunk, subnormal fast range) variant, inlined (as non-inlined calls they raise the kernel's register count
 // enough to halve its occupancy).
 template <int WMB, int CHUNK, bool SUBN>
 __device__ __forceinline__ void g_mfma_tile(const GemmArgs& g, int tm, int tn, int tid,
@@ -126,7 +126,7 @@ __device__ __forceinline__ void grouped_mfma(const GemmArgs& g, int tm, int tn, 
         else g_mfma_tile<WMB, C, false>(g, tm, tn, tid, smem);           \
         return;                                                          \
     }
-    LOF_GM(2) LOF_GM(4) LOF_GM(8) LOF_GM(16) LOF_GM(32)
+    LOF_GM(8) LOF_GM(16) LOF_GM(32)
 #undef LOF_GM
 }
 
@@ -204,8 +204,6 @@ const GroupedConfig kGConfigs[] = {
     LOF_GCFG(64, 64, 4, 4, 16, 8, 1),
     LOF_GCFG(32, 64, 2, 4, 32, 4, 1),
     LOF_GCFG(32, 64, 2, 4, 32, 8, 1),
-    LOF_GCFG(32, 64, 2, 4, 32, 4, 2),
-    LOF_GCFG(32, 64, 2, 4, 32, 8, 2),
 };
 constexpr int kNumGConfigs = sizeof(kGConfigs) / sizeof(kGConfigs[0]);
 int g_default_gconfig = 0;
@@ -262,7 +260,8 @@ lof_status build_task(const lof_group_item& it, const GroupedConfig& cfg, GroupT
     case LOF_ITEM_GEMM:
         s = prepare_gemm(&it.gemm, &t->g);
         if (s != LOF_OK) return s;
-        t->mfma = gemm_chunk_fits_mfma(it.gemm.accum.chunk) ? 1 : 0;
+        t->mfma = (it.gemm.accum.chunk == 8 || it.gemm.accum.chunk == 16 ||
+                   it.gemm.accum.chunk == 32) ? 1 : 0;
         {
             const int bm = t->mfma ? cfg.mfma_bm : cfg.bm;
             const int bn = t->mfma ? 64 : cfg.bn;

@@ -171,6 +171,23 @@ def test_format_accumulators(dev, fmt, kind):
         assert_bits(_run(lof, A, B, acc, dev), _ref(A, B, oacc), f"{fmt} {kind} chunk {chunk}")
 
 
+@pytest.mark.parametrize("kind", ["normal", "tiny", "e4m3_small"])
+@pytest.mark.parametrize("fmt", [FP16_SAT, E4M3])
+def test_format_accumulators_every_config(dev, kind, fmt):
+    """Every tile config that takes the chunk, full-format accumulators whose subnormal range
+    the fast paths cover, on FP32 inputs (products not exact in FP32), subnormal-range sums
+    and e4m3 inputs."""
+    import lofloat_amd as lof
+    A, B = _mats(150, 100, 140, seed=17, kind=kind)
+    for chunk in (1, 4, 32):
+        acc, oacc = _accums(lof, fmt=fmt, chunk=chunk)
+        ref = _ref(A, B, oacc)
+        for cfg in range(lof.gemm_num_configs()):
+            if lof.gemm_config_valid(cfg, chunk):
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref,
+                            f"{fmt} {kind} chunk {chunk} cfg {cfg}")
+
+
 @pytest.mark.parametrize("kind", ["normal", "e4m3", "special", "wide"])
 def test_general_path_forced(dev, kind):
     """config | 0x100 sends every leaf through the general (literal RTO) path."""
