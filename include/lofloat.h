@@ -253,6 +253,64 @@ LOF_API lof_status lof_graph_capture_grouped(lof_grouped* plan, int32_t separate
 LOF_API lof_status lof_graph_launch(lof_graph* graph, lof_stream_t stream);
 LOF_API lof_status lof_graph_destroy(lof_graph* graph);
 
+/* ---------------------------------------------------------------------------------
+ * Bit-pack codec: quantise FP32 to an n-bit rep (1 <= n <= 16) and pack the reps tightly,
+ * or unpack and widen back to FP32 (lo_pack.hpp).  A group of lcm(n, 8) bits (n <= 8) or
+ * lcm(n, 16) bits holds group_count = bits / n elements; fields are LSB-first within
+ * bytes; endian 0 (Forward) puts element i of a group in slot i, 1 (Reverse) in slot
+ * group_count-1-i.  A partial final group is padded with encodings of +0.0f.
+ * ------------------------------------------------------------------------------- */
+enum { LOF_PACK_FORWARD = 0, LOF_PACK_REVERSE = 1 };
+enum { LOF_NEG_TO_ZERO = 0, LOF_NEG_TO_NAN = 1 };   /* Unsigned_behavior (fp_tools.hpp:30-50) */
+
+/* An IsInf / IsNaN checker functor of FloatingPointParams, as data.  The codec reads them
+ * (unlike virtual_round).  kind: 0 matches nothing; 1 rep equals one of codes[0..ncodes);
+ * 2 (rep & a) == a and (rep & b) != 0 (NaN) or == 0 (Inf), the IEEE-style form of
+ * DeviceInfChecker / DeviceNaNChecker (lo_float.h:4386-4398) and the FP16 / BF16 checkers
+ * (fp_tools.hpp:331-396); 3 (rep | a) == codes[0], the P3109 Inf checker
+ * (lo_float.h:1712-1759).  pat0 / pat1: qNanBitPattern / sNanBitPattern of a NaN checker,
+ * minPosInf / minNegInf of an Inf checker. */
+typedef struct lof_checker {
+    int32_t kind;
+    int32_t ncodes;
+    uint32_t codes[3];
+    uint32_t a, b;
+    uint32_t pat0, pat1;
+} lof_checker;
+
+/* FloatingPointParams<IsInf, IsNaN> with the fields the codec reads (fp_tools.hpp:187-252). */
+typedef struct lof_pack_format {
+    lof_format fmt;             /* bitwidth 1..16 */
+    int32_t unsigned_behavior;  /* LOF_NEG_TO_ZERO / LOF_NEG_TO_NAN */
+    lof_checker is_inf, is_nan;
+} lof_pack_format;
+
+/* Bytes a packed buffer of `count` n-bit elements occupies; -1 if n is outside [1, 16].
+ * Replaces LoF_packed_size (lo_pack.hpp:398-402). */
+LOF_API int64_t lof_packed_size(int32_t bitwidth, int64_t count);
+
+/* dst[0 .. lof_packed_size) <- pack(convert_float_to_rep(src[i], fmt, spec)) on the device.
+ * spec.saturation is read (OvfInf / SatFinite / SatPropagate); stochastic modes take one
+ * random word per element (rand_bits, or lof_rand_u32(seed, i) when NULL).  Replaces
+ * LoF_encode_cuda (lo_pack.hpp:496-512) and the CPU LoF_encode(float) (lo_pack.hpp:421-437). */
+LOF_API lof_status lof_pack_encode_f32(uint8_t* dst, const float* src, int64_t count,
+                                       const lof_pack_format* fmt, lof_round_spec spec,
+                                       int32_t endian, const uint32_t* rand_bits, uint64_t seed,
+                                       lof_stream_t stream);
+
+/* dst[i] <- convert_rep_to_float(field i of src).  Replaces LoF_decode_cuda
+ * (lo_pack.hpp:515-530) and the CPU LoF_decode(float) (lo_pack.hpp:457-467). */
+LOF_API lof_status lof_pack_decode_f32(float* dst, const uint8_t* src, int64_t count,
+                                       const lof_pack_format* fmt, int32_t endian,
+                                       lof_stream_t stream);
+
+/* Pack / unpack already-quantised reps (the low n bits of each uint32).  Replace the typed
+ * LoF_encode / LoF_decode of Templated_Float arrays (lo_pack.hpp:405-418, 441-454). */
+LOF_API lof_status lof_pack_reps(uint8_t* dst, const uint32_t* reps, int64_t count,
+                                 int32_t bitwidth, int32_t endian, lof_stream_t stream);
+LOF_API lof_status lof_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t count,
+                                   int32_t bitwidth, int32_t endian, lof_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,17 +3,21 @@
 Drop-in names of the reference package ``LoFloat`` (LoFloat/__init__.py:1-5):
 ``virtual_round``, ``virtual_mx_round``, ``lof_gemm``, ``RoundingMode``, ``InfBehavior``,
 ``NaNBehavior``, ``Signedness``, ``FloatFormatDescriptor``, the ``create_*_params``
-factories, the ``torch.ops.lofloat.*`` operators and the layers.  The arithmetic runs in
+factories, the ``torch.ops.lofloat.*`` operators and the layers; ``lofloat_amd.pack`` is
+the device bit-pack codec of lo_pack.hpp.  The arithmetic runs in
 hand-written HIP kernels in ``liblofloat_hip.so`` (C-ABI: include/lofloat.h); importing
 this package fails if that library has not been built.
 """
 from . import _lib  # noqa: F401  (loads liblofloat_hip.so or raises)
 from .core import (FloatFormatDescriptor, InfBehavior, NaNBehavior, RoundingMode,  # noqa: F401
                    Signedness)
-from .formats import (create_e8m0_params, create_half_params, create_p3109_params,  # noqa: F401
+from .formats import (create_bfloat16_params, create_e8m0_params,  # noqa: F401
+                      create_half_params, create_ocp_e2m1_params, create_ocp_e3m2_params,
+                      create_ocp_e4m3fn_params, create_ocp_e5m2_params, create_p3109_params,
                       create_single_params, make_format)
 from .ops import (AccumSpec, fill_random_u32, gemm, gemm_autotune,  # noqa: F401
-                  gemm_config_name, gemm_config_valid, gemm_num_configs, lof_gemm, set_seed, virtual_mx_round,
+                  gemm_config_name, gemm_config_valid, gemm_num_configs, lof_gemm, set_seed,
+                  virtual_mx_round,
                   virtual_round)
 from . import _custom_ops  # noqa: F401,E402  (registers torch.ops.lofloat.*)
 from ._custom_ops import get_format, register_format  # noqa: F401,E402
@@ -22,6 +26,7 @@ from .layers import (ExplicitMultiheadAttention, FISRBatchNorm, L1BatchNorm,  # 
                      LoFLinearFunction, PWLBatchNorm, PWLSiLU, STEMXRound, STERound, _fwht,
                      _quantize, exp_mant_quantize, mantissa_quantize, replace_mha_with_explicit)
 from . import grouped  # noqa: F401,E402
+from . import pack  # noqa: F401,E402
 from .grouped import GemmJob, GroupedPlan, MXJob, RoundJob  # noqa: F401,E402
 
 __version__ = "0.1.0"

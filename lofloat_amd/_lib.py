@@ -52,6 +52,17 @@ class GroupItem(ctypes.Structure):
                 ("scale_fmt", Format), ("scale_mode", ctypes.c_int32), ("gemm", GemmDesc)]
 
 
+class Checker(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("ncodes", ctypes.c_int32),
+                ("codes", ctypes.c_uint32 * 3), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32),
+                ("pat0", ctypes.c_uint32), ("pat1", ctypes.c_uint32)]
+
+
+class PackFormat(ctypes.Structure):
+    _fields_ = [("fmt", Format), ("unsigned_behavior", ctypes.c_int32),
+                ("is_inf", Checker), ("is_nan", Checker)]
+
+
 # C-ABI symbols and their signatures (include/lofloat.h)
 _P = ctypes.POINTER
 _vp, _i32, _i64, _u64, _f32 = (ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
@@ -71,6 +82,12 @@ SIGNATURES = {
     "lof_gemm_config_name": (_i32, [_i32, ctypes.c_char_p, _i32]),
     "lof_gemm_autotune": (_i32, [_P(GemmDesc), _i32, _vp, _P(_i32), _P(_f32)]),
     "lof_grouped_create": (_i32, [_P(GroupItem), _i32, _i32, _P(_vp)]),
+    "lof_packed_size": (_i64, [_i32, _i64]),
+    "lof_pack_encode_f32": (_i32, [_vp, _vp, _i64, _P(PackFormat), RoundSpec, _i32, _vp, _u64,
+                                   _vp]),
+    "lof_pack_decode_f32": (_i32, [_vp, _vp, _i64, _P(PackFormat), _i32, _vp]),
+    "lof_pack_reps": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp]),
+    "lof_unpack_reps": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp]),
     "lof_grouped_run": (_i32, [_vp, _vp]),
     "lof_grouped_run_separate": (_i32, [_vp, _vp]),
     "lof_grouped_destroy": (_i32, [_vp]),

@@ -43,6 +43,17 @@ class _Accum(ctypes.Structure):
                 ("spec", _Spec), ("chunk", ctypes.c_int32)]
 
 
+class _Checker(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("ncodes", ctypes.c_int32),
+                ("codes", ctypes.c_uint32 * 3), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32),
+                ("pat0", ctypes.c_uint32), ("pat1", ctypes.c_uint32)]
+
+
+class _PackFmt(ctypes.Structure):
+    _fields_ = [("fmt", _Fmt), ("unsigned_behavior", ctypes.c_int32),
+                ("is_inf", _Checker), ("is_nan", _Checker)]
+
+
 def build() -> None:
     """Compile liblof_oracle.so (make -C oracle)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -80,6 +91,25 @@ def _load() -> ctypes.CDLL:
     lib.orc_gemm_f32.argtypes = [i32, i32, i32, f32p, i64, i32, f32p, i64, i32, f32p, i64,
                                  ctypes.c_float, P(_Accum), i64p, i64, i32, f32p, i64,
                                  ctypes.c_float]
+    u8p = P(ctypes.c_uint8)
+    lib.orc_float_to_rep.restype = ctypes.c_uint32
+    lib.orc_float_to_rep.argtypes = [ctypes.c_float, P(_PackFmt), _Spec, ctypes.c_uint32]
+    lib.orc_rep_to_float.restype = ctypes.c_float
+    lib.orc_rep_to_float.argtypes = [ctypes.c_uint32, P(_PackFmt)]
+    lib.orc_packed_bytes.restype = i64
+    lib.orc_packed_bytes.argtypes = [i32, i64]
+    lib.orc_pack_encode.restype = i64
+    lib.orc_pack_encode.argtypes = [u8p, f32p, i64, P(_PackFmt), _Spec, i32, u32p, u64]
+    lib.orc_pack_decode.restype = i64
+    lib.orc_pack_decode.argtypes = [f32p, u8p, i64, P(_PackFmt), i32]
+    lib.orc_pack_reps.restype = i64
+    lib.orc_pack_reps.argtypes = [u8p, u32p, i64, i32, i32]
+    lib.orc_unpack_reps.restype = i64
+    lib.orc_unpack_reps.argtypes = [u32p, u8p, i64, i32, i32]
+    lib.orc_float_to_rep_array.restype = None
+    lib.orc_float_to_rep_array.argtypes = [f32p, u32p, i64, P(_PackFmt), _Spec, u32p, u64]
+    lib.orc_rep_to_float_array.restype = None
+    lib.orc_rep_to_float_array.argtypes = [u32p, f32p, i64, P(_PackFmt)]
     lib.orc_max_threads.restype = i32
     return lib
 
@@ -228,4 +258,110 @@ def gemm(A, B, accum: _Accum, trans_a: bool = False, trans_b: bool = False,
                       _ptr(B, ctypes.c_float), ldb, int(trans_b), _ptr(out, ctypes.c_float), N,
                       float(alpha), ctypes.byref(accum), None, 0, nthreads,
                       _ptr(Cc, ctypes.c_float), N, float(beta))
+    return out
+
+
+# ------------------------------------------------------------------ bit-pack codec
+# Checker presets: the Inf / NaN functors of the reference's format parameter sets, as data
+# (kind, codes, a, b, pat0, pat1); see orc_checker in lof_oracle.h.
+def _chk(kind=0, codes=(), a=0, b=0, pat0=0, pat1=0) -> _Checker:
+    c = list(codes) + [0] * (3 - len(codes))
+    return _Checker(kind, len(codes), (ctypes.c_uint32 * 3)(*c), a, b, pat0, pat1)
+
+
+def checkers(preset, fmt=None):
+    """(is_inf, is_nan) checker pair for a preset name or tuple:
+    'ocp_fn' (OCP_F8E4M3_*: NaN == 0xFF, no Inf; lo_float.h:1448-1483), ('p3109', k, signed,
+    has_inf) (lo_float.h:1678-1759), ('ieee_f8', signed) (1485-1519), 'fp16' / 'bf16'
+    (fp_tools.hpp:331-396), 'e8m0' (lo_float.h:1777-1789), 'trivial' (1411-1446), 'device'
+    (DeviceInf/NaNChecker, lo_float.h:4386-4398, masks from fmt; patterns default 0)."""
+    if preset == "ocp_fn":
+        return _chk(), _chk(1, (0xFF,), pat0=0xFF, pat1=0xFF)
+    if preset == "e8m0":
+        return _chk(), _chk(1, (0xFF,), pat0=0xFF, pat1=0xFF)
+    if preset == "trivial":
+        return _chk(), _chk()
+    if preset == "fp16":
+        return (_chk(2, (), 0x7C00, 0x3FF, 0x7C00, 0xFC00),
+                _chk(2, (), 0x7C00, 0x3FF, 0x7E00, 0x7C01))
+    if preset == "bf16":
+        return (_chk(2, (), 0x7F80, 0x7F, 0x7F80, 0xFF80),
+                _chk(2, (), 0x7F80, 0x7F, 0x7FC0, 0x7F81))
+    if preset == "device":
+        n, mb = fmt[0], fmt[1]
+        em, mm = ((1 << (n - mb - 1)) - 1) << mb, (1 << mb) - 1
+        return _chk(2, (), em, mm), _chk(2, (), em, mm)
+    kind = preset[0]
+    if kind == "p3109":
+        _, k, signed, has_inf = preset
+        nan = (1 << (k - 1)) if signed else (1 << k) - 1
+        pos = (1 << (k - 1)) - 1 if signed else (1 << k) - 2
+        neg = (1 << k) - 1 if signed else 0
+        target = (1 << k) - 1 if signed else (1 << k) - 2
+        inf = _chk(3, (target,), 1 << (k - 1), 0, pos, neg) if has_inf else _chk(0, (), 0, 0, pos, neg)
+        return inf, _chk(1, (nan,), pat0=nan, pat1=nan)
+    if kind == "ieee_f8":
+        nan = 0x80 if preset[1] else 0xFF
+        return _chk(1, (0x7F, 0xFF), pat0=0x7F, pat1=0xFF), _chk(1, (nan,), pat0=nan, pat1=nan)
+    raise ValueError(f"unknown checker preset {preset!r}")
+
+
+def pack_format(fmt, preset, unsigned_behavior: int = 0) -> _PackFmt:
+    inf, nan = checkers(preset, fmt)
+    return _PackFmt(_fmt(fmt), int(unsigned_behavior), inf, nan)
+
+
+def packed_bytes(n: int, count: int) -> int:
+    return int(_lib.orc_packed_bytes(n, count))
+
+
+def float_to_rep(x, pf: _PackFmt, mode: int = RNE, saturation: int = 0, stoch_len: int = 0,
+                 bits=None, seed: int = 0) -> np.ndarray:
+    x = _f32(x)
+    out = np.empty(x.size, np.uint32)
+    b = None if bits is None else np.ascontiguousarray(bits, dtype=np.uint32)
+    _lib.orc_float_to_rep_array(_ptr(x, ctypes.c_float), _ptr(out, ctypes.c_uint32), x.size,
+                                ctypes.byref(pf), _Spec(mode, saturation, stoch_len),
+                                _ptr(b, ctypes.c_uint32), seed)
+    return out
+
+
+def rep_to_float(reps, pf: _PackFmt) -> np.ndarray:
+    r = np.ascontiguousarray(reps, dtype=np.uint32)
+    out = np.empty(r.size, np.float32)
+    _lib.orc_rep_to_float_array(_ptr(r, ctypes.c_uint32), _ptr(out, ctypes.c_float), r.size,
+                                ctypes.byref(pf))
+    return out
+
+
+def pack_encode(x, pf: _PackFmt, mode: int = RNE, saturation: int = 0, stoch_len: int = 0,
+                endian: int = 0, bits=None, seed: int = 0) -> np.ndarray:
+    x = _f32(x)
+    out = np.zeros(packed_bytes(pf.fmt.bitwidth, x.size), np.uint8)
+    b = None if bits is None else np.ascontiguousarray(bits, dtype=np.uint32)
+    _lib.orc_pack_encode(_ptr(out, ctypes.c_uint8), _ptr(x, ctypes.c_float), x.size,
+                         ctypes.byref(pf), _Spec(mode, saturation, stoch_len), endian,
+                         _ptr(b, ctypes.c_uint32), seed)
+    return out
+
+
+def pack_decode(buf, count: int, pf: _PackFmt, endian: int = 0) -> np.ndarray:
+    b = np.ascontiguousarray(buf, dtype=np.uint8)
+    out = np.empty(count, np.float32)
+    _lib.orc_pack_decode(_ptr(out, ctypes.c_float), _ptr(b, ctypes.c_uint8), count,
+                         ctypes.byref(pf), endian)
+    return out
+
+
+def pack_reps(reps, n: int, endian: int = 0) -> np.ndarray:
+    r = np.ascontiguousarray(reps, dtype=np.uint32)
+    out = np.zeros(packed_bytes(n, r.size), np.uint8)
+    _lib.orc_pack_reps(_ptr(out, ctypes.c_uint8), _ptr(r, ctypes.c_uint32), r.size, n, endian)
+    return out
+
+
+def unpack_reps(buf, count: int, n: int, endian: int = 0) -> np.ndarray:
+    b = np.ascontiguousarray(buf, dtype=np.uint8)
+    out = np.empty(count, np.uint32)
+    _lib.orc_unpack_reps(_ptr(out, ctypes.c_uint32), _ptr(b, ctypes.c_uint8), count, n, endian)
     return out

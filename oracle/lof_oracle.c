@@ -361,3 +361,263 @@ void orc_gemm_f32(int M, int N, int K, const float* A, int64_t lda, int trans_a,
             D[(int64_t)i * ldd + j] = gemm_one(i, j, K, A, lda, trans_a, B, ldb, trans_b,
                                                alpha, ac, C, ldc, beta);
 }
+
+/* ============================================================== bit-pack codec */
+
+/* RoundMantissa on a 64-bit word (lo_float.h:1232-1272 with Bits = uint64_t); random
+ * word u mapped as in orc_round_mantissa_bits. */
+static uint64_t round_bits64(uint64_t bits, int r, int mode, int len, int pos, uint32_t u) {
+    const uint64_t low = r >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << r) - 1u);
+    const uint64_t T = bits & ~low, tail = bits & low, one_r = (uint64_t)1 << r;
+    const uint64_t R = len > 0 ? (uint64_t)(u & low_mask(len)) : 0u;
+    switch (mode) {
+    case ORC_RNE: return r == 0 ? bits : bits + ((bits >> r) & 1u) + ((uint64_t)1 << (r - 1)) - 1u;
+    case ORC_RTZ: return T;
+    case ORC_RAFZ: return T + (tail ? one_r : 0u);
+    case ORC_SRA: return len <= 0 ? bits : bits + (R << (r - len));
+    case ORC_RNO: return r == 0 ? bits : bits + ((~bits >> r) & 1u) + ((uint64_t)1 << (r - 1)) - 1u;
+    case ORC_RDOWN: return T + ((!pos && tail) ? one_r : 0u);
+    case ORC_RUP: return T + ((pos && tail) ? one_r : 0u);
+    case ORC_RTA: return r == 0 ? bits : T + (((bits >> (r - 1)) & 1u) << r);
+    case ORC_SRB: return bits + (tail > 0 ? ((R + 1u) << (r - len)) : 0u);
+    case ORC_SRC:
+        if (len == 0 || tail == 0) return bits;
+        return bits + (((R >> 1) + (R & 1u)) << (r - len + 1));
+    case ORC_TRUESR: {
+        const double prob = (double)tail / ldexp(1.0, r);
+        const double samp = (double)u * (1.0 / 4294967296.0);
+        return samp < prob ? T + one_r : T;
+    }
+    case ORC_PROB: return T + (tail ? ((uint64_t)(u & 1u) << r) : 0u);
+    case ORC_RTO: return T | (tail ? one_r : 0u);
+    case ORC_RTTZ: return r == 0 ? bits : bits + ((uint64_t)1 << (r - 1)) - 1u;
+    case ORC_SRD: {
+        if (len <= 0) return bits;
+        return bits + ((R + ((u >> len) & 1u)) << (r - len));
+    }
+    default: return bits;
+    }
+}
+
+/* Stochastic rounding of bits < 2^24 at a shift r > 24 (lo_pack.hpp:199-220): the result is
+ * 0 or 1.  The reference shifts a uint64_t by r - len there, undefined once r >= 64; here
+ * the result is the exact integer (bits + increment) >> r, i.e. 1 iff
+ * k + (bits >> (r - len)) >= 2^len for an increment k * 2^(r - len). */
+static uint32_t stoch_wide(uint64_t bits, int r, int mode, int len, uint32_t u) {
+    const uint64_t R = len > 0 ? (uint64_t)(u & low_mask(len)) : 0u;
+    const int s = r - len;
+    const uint64_t hi = s >= 64 ? 0u : (bits >> s);
+    const uint64_t full = (uint64_t)1 << len;
+    switch (mode) {
+    case ORC_SRA: return (len > 0 && R + hi >= full) ? 1u : 0u;
+    case ORC_SRB: return (bits > 0 && (R + 1u) + hi >= full) ? 1u : 0u;
+    case ORC_SRC: {
+        if (len == 0 || bits == 0) return 0u;
+        const int s1 = r - len + 1;
+        const uint64_t hi1 = s1 >= 64 ? 0u : (bits >> s1);
+        return ((R >> 1) + (R & 1u) + hi1 >= ((uint64_t)1 << (len - 1))) ? 1u : 0u;
+    }
+    case ORC_SRD: return (len > 0 && R + ((u >> len) & 1u) + hi >= full) ? 1u : 0u;
+    case ORC_TRUESR: {   /* up iff u * 2^r < bits * 2^32, bits < 2^24 */
+        if (bits == 0) return 0u;
+        if (r <= 32) return ((uint64_t)u < (bits << (32 - r))) ? 1u : 0u;
+        if (r - 32 >= 24) return u == 0 ? 1u : 0u;
+        return (((uint64_t)u << (r - 32)) < bits) ? 1u : 0u;
+    }
+    case ORC_PROB: return bits ? (u & 1u) : 0u;
+    default: return 0u;
+    }
+}
+
+static int chk_match(const orc_checker* c, uint32_t rep, int nan) {
+    switch (c->kind) {
+    case 1:
+        for (int i = 0; i < c->ncodes && i < 3; ++i)
+            if (rep == c->codes[i]) return 1;
+        return 0;
+    case 2: return (rep & c->a) == c->a && (nan ? (rep & c->b) != 0 : (rep & c->b) == 0);
+    case 3: return (rep | c->a) == c->codes[0];
+    default: return 0;
+    }
+}
+
+uint32_t orc_float_to_rep(float from, const orc_pack_format* P, orc_spec ps, uint32_t u) {
+    const orc_format* F = &P->fmt;
+    const int n = F->bitwidth, mb = F->mantissa_bits, bias = F->bias;
+    const int is_signed = F->signedness == 0;
+    const int exp_bits = is_signed ? n - mb - 1 : n - mb;
+    const int exp_off = bias - 127;                       /* kExponentOffset */
+    const int digit_shift = mb - 23;                      /* kDigitShift (< 0 here) */
+    const int to_min_exp = 1 - bias;
+    const int to_max_exp = (1 << exp_bits) - 1 - bias;
+    const int has_inf = F->inf_behavior != 1;             /* not Saturating */
+    const uint32_t signmask = is_signed ? (1u << (n - 1)) : 0u;
+    const uint32_t max_finite = has_inf ? P->is_inf.pat0 - 1u
+                              : (is_signed ? (((1u << n) - 1u) >> 1) : ((1u << n) - 1u));
+    const uint32_t inf_rep = has_inf ? P->is_inf.pat0 : max_finite;
+    const uint32_t xb = f2u(from);
+    const int neg = (int)(xb >> 31);
+    const uint32_t from_bits = xb & 0x7FFFFFFFu;
+#define WITH_SIGN(mag) \
+    ((!neg || !is_signed) ? (mag) : (((mag) == 0u && chk_match(&P->is_nan, signmask, 1)) ? 0u : ((mag) | signmask)))
+
+    if (!is_signed && neg) {                                  /* lo_pack.hpp:161-166 */
+        if (P->unsigned_behavior == 0) return 0u;
+        return F->nan_behavior == 0 ? P->is_nan.pat0 : P->is_nan.pat1;
+    }
+    if (isinf(from)) {                                        /* 170-174 */
+        if (ps.saturation != 1 && has_inf) return WITH_SIGN(inf_rep);
+        return WITH_SIGN(max_finite);
+    }
+    if (isnan(from)) return P->is_nan.pat0;                   /* 175 */
+    if (from_bits == 0) return WITH_SIGN(0u);                 /* 176 */
+    const int bfe = (int)(from_bits >> 23);
+
+    if (to_min_exp < -125 && bfe == 0) {                      /* 181-198: float subnormal in */
+        uint64_t bits = from_bits;
+        const int nf = __builtin_clz(from_bits) - (32 - 23) + 1;
+        const int be = exp_off - nf + 1;
+        if (be <= 0) {
+            if (exp_off < 64) bits <<= exp_off;
+        } else {
+            bits <<= nf;
+            bits &= ~((uint64_t)1 << 23);
+            bits |= (uint64_t)be << 23;
+        }
+        bits = round_bits64(bits, -digit_shift, ps.mode, ps.stoch_len, 0, u);
+        bits >>= -digit_shift;
+        return WITH_SIGN((uint32_t)bits);
+    }
+    if (to_min_exp > -125) {                                  /* 201-239: to subnormal / zero */
+        const int bte = bfe - 127 + bias;
+        if (bte <= 0) {
+            const uint32_t lead = bfe > 0 ? 1u : 0u;
+            const int shift = -digit_shift - bte + (int)lead;
+            const uint32_t rfb = (from_bits & 0x7FFFFFu) | (lead << 23);
+            uint32_t bits;
+            if (shift <= 23 + 1) {
+                bits = (uint32_t)(round_bits64(rfb, shift, ps.mode, ps.stoch_len, 0, u) >> shift);
+            } else {
+                switch (ps.mode) {
+                case ORC_RTO: case ORC_RAFZ: bits = rfb > 0 ? 1u : 0u; break;
+                case ORC_RTZ: bits = 0u; break;
+                case ORC_RUP: bits = (rfb && !neg) ? 1u : 0u; break;
+                case ORC_RDOWN: bits = (rfb && neg) ? 1u : 0u; break;
+                case ORC_SRA: case ORC_SRB: case ORC_SRC: case ORC_SRD: case ORC_TRUESR: case ORC_PROB:
+                    bits = stoch_wide(rfb, shift, ps.mode, ps.stoch_len, u); break;
+                default: bits = 0u; break;
+                }
+            }
+            return WITH_SIGN(bits);
+        }
+    }
+    /* normal path, 242-257 */
+    uint64_t rb = from_bits;
+    rb = round_bits64(rb, -digit_shift, ps.mode, ps.stoch_len, 0, u);
+    rb &= ~((((uint64_t)1) << (-digit_shift)) - 1u);
+    rb += (uint64_t)(int64_t)exp_off << 23;
+    const uint64_t aligned_highest = (uint64_t)max_finite << (-digit_shift);
+    const uint32_t bits = (uint32_t)(rb >> (-digit_shift));
+    const int narrower = to_max_exp < 128 || (to_max_exp == 128 && mb + 1 < 24);
+    if (narrower && rb > aligned_highest) {
+        const int emit_inf = ps.saturation == 0 && has_inf;
+        return WITH_SIGN(emit_inf ? inf_rep : max_finite);
+    }
+    return WITH_SIGN(bits);
+#undef WITH_SIGN
+}
+
+float orc_rep_to_float(uint32_t rep, const orc_pack_format* P) {
+    const orc_format* F = &P->fmt;
+    const int n = F->bitwidth, mb = F->mantissa_bits, bias = F->bias;
+    const int is_signed = F->signedness == 0;
+    rep &= n >= 32 ? 0xFFFFFFFFu : ((1u << n) - 1u);
+    const int neg = is_signed ? (int)((rep >> (n - 1)) & 1u) : 0;
+    const uint32_t mag = is_signed ? (rep & ((1u << (n - 1)) - 1u)) : rep;
+    if (F->inf_behavior != 1 && chk_match(&P->is_inf, rep, 0)) return neg ? -INFINITY : INFINITY;
+    if (chk_match(&P->is_nan, rep, 1)) return u2f(0x7FC00000u);
+    const uint32_t e = mag >> mb, m = mag & ((1u << mb) - 1u);
+    const double v = e == 0 ? ldexp((double)m, 1 - bias - mb)
+                            : ldexp(1.0 + (double)m / (double)(1u << mb), (int)e - bias);
+    return (float)(neg ? -v : v);
+}
+
+static int pk_gcd(int a, int b) { return b == 0 ? a : pk_gcd(b, a % b); }
+static int pk_group_bits(int n) { const int al = n <= 8 ? 8 : 16; return n / pk_gcd(n, al) * al; }
+
+int64_t orc_packed_bytes(int n, int64_t count) {
+    const int gbits = pk_group_bits(n), gc = gbits / n, gb = gbits / 8;
+    return (count + gc - 1) / gc * gb;
+}
+
+static inline int64_t slot_bit(int64_t i, int n, int gc, int gbits, int endian) {
+    const int64_t g = i / gc;
+    const int j = (int)(i % gc);
+    const int slot = endian ? gc - 1 - j : j;
+    return g * gbits + (int64_t)slot * n;
+}
+
+static void put_field(uint8_t* dst, int64_t off, uint32_t v, int n) {
+    for (int b = 0; b < n; ++b) {
+        const int64_t p = off + b;
+        dst[p >> 3] |= (uint8_t)(((v >> b) & 1u) << (p & 7));
+    }
+}
+static uint32_t get_field(const uint8_t* src, int64_t off, int n) {
+    uint32_t v = 0;
+    for (int b = 0; b < n; ++b) {
+        const int64_t p = off + b;
+        v |= (uint32_t)((src[p >> 3] >> (p & 7)) & 1u) << b;
+    }
+    return v;
+}
+
+int64_t orc_pack_reps(uint8_t* dst, const uint32_t* reps, int64_t count, int n, int endian) {
+    const int gbits = pk_group_bits(n), gc = gbits / n;
+    const int64_t bytes = orc_packed_bytes(n, count), padded = (count + gc - 1) / gc * gc;
+    const uint32_t mask = (1u << n) - 1u;
+    memset(dst, 0, (size_t)bytes);
+    for (int64_t i = 0; i < padded; ++i)
+        put_field(dst, slot_bit(i, n, gc, gbits, endian), i < count ? (reps[i] & mask) : 0u, n);
+    return bytes;
+}
+
+int64_t orc_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t count, int n, int endian) {
+    const int gbits = pk_group_bits(n), gc = gbits / n;
+    for (int64_t i = 0; i < count; ++i) reps[i] = get_field(src, slot_bit(i, n, gc, gbits, endian), n);
+    return orc_packed_bytes(n, count);
+}
+
+int64_t orc_pack_encode(uint8_t* dst, const float* src, int64_t count,
+                        const orc_pack_format* f, orc_spec ps, int endian,
+                        const uint32_t* bits, uint64_t seed) {
+    const int n = f->fmt.bitwidth;
+    const int gbits = pk_group_bits(n), gc = gbits / n;
+    const int64_t bytes = orc_packed_bytes(n, count), padded = (count + gc - 1) / gc * gc;
+    memset(dst, 0, (size_t)bytes);
+    for (int64_t i = 0; i < padded; ++i) {
+        /* the final partial group is padded with 0.0f (lo_pack.hpp:427-431) */
+        const float x = i < count ? src[i] : 0.0f;
+        const uint32_t u = i < count ? word_at(bits, seed, i) : 0u;
+        const uint32_t rep = orc_float_to_rep(x, f, ps, u) & ((1u << n) - 1u);
+        put_field(dst, slot_bit(i, n, gc, gbits, endian), rep, n);
+    }
+    return bytes;
+}
+
+int64_t orc_pack_decode(float* dst, const uint8_t* src, int64_t count,
+                        const orc_pack_format* f, int endian) {
+    const int n = f->fmt.bitwidth;
+    const int gbits = pk_group_bits(n), gc = gbits / n;
+    for (int64_t i = 0; i < count; ++i)
+        dst[i] = orc_rep_to_float(get_field(src, slot_bit(i, n, gc, gbits, endian), n), f);
+    return orc_packed_bytes(n, count);
+}
+
+void orc_float_to_rep_array(const float* x, uint32_t* out, int64_t n, const orc_pack_format* f,
+                            orc_spec ps, const uint32_t* bits, uint64_t seed) {
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_float_to_rep(x[i], f, ps, word_at(bits, seed, i));
+}
+void orc_rep_to_float_array(const uint32_t* rep, float* out, int64_t n, const orc_pack_format* f) {
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_rep_to_float(rep[i], f);
+}

@@ -96,6 +96,52 @@ void orc_gemm_f32(int M, int N, int K, const float* A, int64_t lda, int trans_a,
                   float alpha, const orc_accum* acc, const int64_t* samples,
                   int64_t n_samples, int nthreads, const float* C, int64_t ldc, float beta);
 
+/* ---------------------------------------------------------------- bit-pack codec
+ * lo_pack.hpp: n-bit floats (1 <= n <= 16) packed in groups of lcm(n, 8) (n <= 8) or
+ * lcm(n, 16) bits, LSB-first within bytes; endian 0 = Forward (element i in slot i),
+ * 1 = Reverse (slot G-1-i). */
+
+/* An Inf or NaN checker functor of FloatingPointParams, as data (lo_float.h:1411-1800,
+ * fp_tools.hpp:255-400, lo_float.h:4386-4398).  kind: 0 never matches; 1 rep equals one
+ * of codes[0 .. ncodes); 2 (rep & a) == a and (rep & b) != 0 (NaN) / == 0 (Inf);
+ * 3 (rep | a) == codes[0].  pat0 / pat1: qNanBitPattern / sNanBitPattern of a NaN
+ * checker, minPosInf / minNegInf of an Inf checker. */
+typedef struct orc_checker {
+    int32_t kind;
+    int32_t ncodes;
+    uint32_t codes[3];
+    uint32_t a, b;
+    uint32_t pat0, pat1;
+} orc_checker;
+
+typedef struct orc_pack_format {
+    orc_format fmt;
+    int32_t unsigned_behavior;   /* 0 NegtoZero, 1 NegtoNaN (fp_tools.hpp:30-50) */
+    orc_checker is_inf, is_nan;
+} orc_pack_format;
+
+/* convert_float_to_rep (lo_pack.hpp:108-258); ps.saturation is read here. */
+uint32_t orc_float_to_rep(float x, const orc_pack_format* f, orc_spec ps, uint32_t u);
+/* convert_rep_to_float (lo_pack.hpp:268-304). */
+float orc_rep_to_float(uint32_t rep, const orc_pack_format* f);
+/* LoF_packed_size (lo_pack.hpp:398-402). */
+int64_t orc_packed_bytes(int n, int64_t count);
+/* LoF_encode(float) / LoF_decode(float) (lo_pack.hpp:421-467); bits == NULL -> words
+ * orc_rand_u32(seed, i) for element i.  Return bytes written / consumed. */
+int64_t orc_pack_encode(uint8_t* dst, const float* src, int64_t count,
+                        const orc_pack_format* f, orc_spec ps, int endian,
+                        const uint32_t* bits, uint64_t seed);
+int64_t orc_pack_decode(float* dst, const uint8_t* src, int64_t count,
+                        const orc_pack_format* f, int endian);
+/* LoF_encode / LoF_decode of typed values (lo_pack.hpp:405-418, 441-454): raw reps. */
+int64_t orc_pack_reps(uint8_t* dst, const uint32_t* reps, int64_t count, int n, int endian);
+int64_t orc_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t count, int n, int endian);
+
+/* Elementwise forms of the two codec functions (random words as for orc_pack_encode). */
+void orc_float_to_rep_array(const float* x, uint32_t* out, int64_t n, const orc_pack_format* f,
+                            orc_spec ps, const uint32_t* bits, uint64_t seed);
+void orc_rep_to_float_array(const uint32_t* rep, float* out, int64_t n, const orc_pack_format* f);
+
 int orc_max_threads(void);
 
 #ifdef __cplusplus
