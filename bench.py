@@ -10,7 +10,8 @@ with e2m1 and with SR-A random words; the emulated-accumulation GEMM at M=N=K=40
 (TFLOP/s = 2MNK / t) for 10-bit mantissa and 5e10m accumulators at chunks 1/8/32; one
 LoF_Linear forward+backward step (X 8192x4096, W 4096x4096); and the grouped launch of 64
 uneven rounding jobs + 12 GEMMs (e4m3 operands) against per-job launches, both captured in
-HIP graphs.
+HIP graphs; the bit-pack codec encode / decode of 2^28 values to e4m3fn and to an 11-bit
+P3109 format.
 
 The CPU baseline is the C oracle (a restatement of the reference's scalar semantics) run
 with OpenMP on the host cores on the same workload.
@@ -74,10 +75,11 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run N device copies of the 8192x8192 matrix (a plain copy "
                          "kernel of known traffic, to calibrate HBM byte counters)")
-    ap.add_argument("--only", choices=["gemm", "linear", "grouped"], default=None,
+    ap.add_argument("--only", choices=["gemm", "linear", "grouped", "pack"], default=None,
                     help="run one workload alone (for profiling): 'gemm' = the 4096^3 "
                          "emulated-accumulation GEMM with --chunk / --accum / --inputs; "
-                         "'linear' = the LoF_Linear step; 'grouped' = the batched launch")
+                         "'linear' = the LoF_Linear step; 'grouped' = the batched launch; "
+                         "'pack' = the bit-pack codec encode / decode of 2^28 elements")
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--accum", choices=["mant10", "5e10m"], default="5e10m")
     ap.add_argument("--inputs", choices=["e4m3", "8e7m"], default="e4m3")
@@ -101,10 +103,11 @@ def main():
     if args.only == "gemm":
         gemm_only(lof, torch, dev, args)
         return
-    if args.only in ("linear", "grouped"):
+    if args.only in ("linear", "grouped", "pack"):
         import numpy as np
         out = {}
-        (linear_steps if args.only == "linear" else grouped_vs_separate)(lof, torch, np, dev, out)
+        {"linear": linear_steps, "grouped": grouped_vs_separate,
+         "pack": pack_codec}[args.only](lof, torch, np, dev, out)
         print(json.dumps(out), flush=True)
         return
 
@@ -253,7 +256,42 @@ def extras(lof, torch, np, oracle, dev):
 
     linear_steps(lof, torch, np, dev, out)
     grouped_vs_separate(lof, torch, np, dev, out)
+    pack_codec(lof, torch, np, dev, out)
     return out
+
+
+def pack_codec(lof, torch, np, dev, out):
+    """Bit-pack codec: encode (round + pack) and decode (unpack + widen) of 2^28 FP32 values
+    (1 GiB, four times the MALL) to e4m3fn (8 bits) and to P3109 binary11p5 (11 bits, groups
+    of 16 in 22 bytes), RNE.  Algorithmic bytes per element: 4 + n/8 both ways."""
+    n = 1 << 28
+    g = torch.Generator(device=dev).manual_seed(7)
+    x = torch.randn(n, device=dev, generator=g)
+    y = torch.empty_like(x)
+    fmts = (("e4m3fn", lof.create_ocp_e4m3fn_params()), ("p3109_11p5", lof.create_p3109_params(11, 5)))
+    for name, fmt in fmts:
+        nbits = fmt.total_bits
+        buf = torch.empty(lof.pack.packed_size(nbits, n), dtype=torch.uint8, device=dev)
+        algo = n * (4 + nbits / 8)
+        ms = timed(lambda i: lof.pack.encode(x, fmt, out=buf), 10)
+        out[f"pack_encode_2^28_{name}_GBs"] = round(algo / ms / 1e6, 1)
+        ms_d = timed(lambda i: lof.pack.decode(buf, fmt, n, out=y), 10)
+        out[f"pack_decode_2^28_{name}_GBs"] = round(algo / ms_d / 1e6, 1)
+        log(f"pack {name}: encode {ms * 1e3:.0f} us ({algo / ms / 1e6:.0f} GB/s), "
+            f"decode {ms_d * 1e3:.0f} us ({algo / ms_d / 1e6:.0f} GB/s)")
+        del buf
+    # raw reps (no rounding): the packing alone, 4 + n/8 bytes per element both ways
+    reps = torch.randint(0, 1 << 11, (n,), dtype=torch.int32, device=dev)
+    ms = timed(lambda i: lof.pack.pack_reps(reps, 11), 10)
+    buf = lof.pack.pack_reps(reps, 11)
+    ms_d = timed(lambda i: lof.pack.unpack_reps(buf, 11, n), 10)
+    algo = n * (4 + 11 / 8)
+    out["pack_reps_2^28_n11_GBs"] = round(algo / ms / 1e6, 1)
+    out["unpack_reps_2^28_n11_GBs"] = round(algo / ms_d / 1e6, 1)
+    log(f"pack_reps n=11: {ms * 1e3:.0f} us, unpack {ms_d * 1e3:.0f} us")
+    del reps, buf
+    del x, y
+    torch.cuda.empty_cache()
 
 
 def linear_steps(lof, torch, np, dev, out):

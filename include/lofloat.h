@@ -286,26 +286,26 @@ typedef struct lof_pack_format {
 } lof_pack_format;
 
 /* Bytes a packed buffer of `count` n-bit elements occupies; -1 if n is outside [1, 16].
- * Replaces LoF_packed_size (lo_pack.hpp:398-402). */
+ * Replaces LoF_packed_size (lo_pack.hpp:392-394). */
 LOF_API int64_t lof_packed_size(int32_t bitwidth, int64_t count);
 
 /* dst[0 .. lof_packed_size) <- pack(convert_float_to_rep(src[i], fmt, spec)) on the device.
  * spec.saturation is read (OvfInf / SatFinite / SatPropagate); stochastic modes take one
  * random word per element (rand_bits, or lof_rand_u32(seed, i) when NULL).  Replaces
- * LoF_encode_cuda (lo_pack.hpp:496-512) and the CPU LoF_encode(float) (lo_pack.hpp:421-437). */
+ * LoF_encode_cuda (lo_pack.hpp:504-517) and the CPU LoF_encode(float) (lo_pack.hpp:414-429). */
 LOF_API lof_status lof_pack_encode_f32(uint8_t* dst, const float* src, int64_t count,
                                        const lof_pack_format* fmt, lof_round_spec spec,
                                        int32_t endian, const uint32_t* rand_bits, uint64_t seed,
                                        lof_stream_t stream);
 
 /* dst[i] <- convert_rep_to_float(field i of src).  Replaces LoF_decode_cuda
- * (lo_pack.hpp:515-530) and the CPU LoF_decode(float) (lo_pack.hpp:457-467). */
+ * (lo_pack.hpp:519-531) and the CPU LoF_decode(float) (lo_pack.hpp:448-463). */
 LOF_API lof_status lof_pack_decode_f32(float* dst, const uint8_t* src, int64_t count,
                                        const lof_pack_format* fmt, int32_t endian,
                                        lof_stream_t stream);
 
 /* Pack / unpack already-quantised reps (the low n bits of each uint32).  Replace the typed
- * LoF_encode / LoF_decode of Templated_Float arrays (lo_pack.hpp:405-418, 441-454). */
+ * LoF_encode / LoF_decode of Templated_Float arrays (lo_pack.hpp:397-412, 431-446). */
 LOF_API lof_status lof_pack_reps(uint8_t* dst, const uint32_t* reps, int64_t count,
                                  int32_t bitwidth, int32_t endian, lof_stream_t stream);
 LOF_API lof_status lof_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t count,

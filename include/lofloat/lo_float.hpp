@@ -7,6 +7,8 @@
 //   run_virtual_mx_round                          src/Lof_kernels.h:44-48, Lof_kernel.cu:70-94
 //   Layout, Matrix<T, idx, Layout>                src/layouts.h:16-19, src/Matrix.h:21-78
 //   Gemm<MatA, MatB, MatC, MatD>                  src/cutlass_gemms.cuh:1629-1896
+//   PackEndian, LoF_packed_size, LoF_encode_cuda, LoF_decode_cuda and the checker
+//   functors they read                           src/lo_pack.hpp:50, 392-531; lo_float.h:1411-1789
 // Same names, argument meanings and enum values.  Differences: every launcher takes an
 // optional hipStream_t (default: the legacy default stream, as the reference launches), an
 // optional per-element random-word array and seed for the stochastic modes, and reports
@@ -18,6 +20,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <initializer_list>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -296,5 +299,203 @@ private:
         return d;
     }
 };
+
+// ---------------------------------------------------------------------------------------
+// n-bit pack codec (lo_pack.hpp:50, 392-531).
+//
+// The codec reads the Inf / NaN checker functors of FloatingPointParams.  A functor crosses
+// the C-ABI as a lof_checker description, which it supplies through a static lof()
+// member.  The checkers below are the reference's (lo_float.h:1411-1789,
+// fp_tools.hpp:331-396) with that member; NoChecker matches nothing.  A user-defined
+// checker passes the codec only once it adds lof().
+enum class PackEndian { Forward, Reverse };
+
+namespace detail {
+constexpr lof_checker make_checker(int32_t kind, std::initializer_list<uint32_t> codes,
+                                   uint32_t a, uint32_t b, uint32_t pat0, uint32_t pat1) {
+    lof_checker c{kind, (int32_t)codes.size(), {0u, 0u, 0u}, a, b, pat0, pat1};
+    int i = 0;
+    for (uint32_t v : codes) c.codes[i++] = v;
+    return c;
+}
+}  // namespace detail
+
+struct Trivial_NaNChecker {
+    bool operator()(uint32_t) const { return false; }
+    uint32_t qNanBitPattern() const { return 0; }
+    uint32_t sNanBitPattern() const { return 0; }
+    static constexpr lof_checker lof() { return detail::make_checker(0, {}, 0, 0, 0, 0); }
+};
+struct Trivial_InfChecker {
+    bool operator()(uint32_t) const { return false; }
+    uint32_t minNegInf() const { return 0; }
+    uint32_t minPosInf() const { return 0; }
+    static constexpr lof_checker lof() { return detail::make_checker(0, {}, 0, 0, 0, 0); }
+};
+// e4m3fn / e5m2 (as the reference's param_float8_e5m2 uses it): NaN is 0xFF, no Inf.
+struct OCP_F8E4M3_NaNChecker {
+    bool operator()(uint32_t bits) const { return bits == 0xFFu; }
+    uint32_t qNanBitPattern() const { return 0xFF; }
+    uint32_t sNanBitPattern() const { return 0xFF; }
+    static constexpr lof_checker lof() { return detail::make_checker(1, {0xFFu}, 0, 0, 0xFF, 0xFF); }
+};
+struct OCP_F8E4M3_InfChecker {
+    bool operator()(uint32_t) const { return false; }
+    uint32_t minNegInf() const { return 0; }
+    uint32_t minPosInf() const { return 0; }
+    static constexpr lof_checker lof() { return detail::make_checker(0, {}, 0, 0, 0, 0); }
+};
+struct OCP_F8E5M2_NaNChecker {
+    bool operator()(uint32_t bits) const { return bits == 0xFFu || bits == 0xFEu || bits == 0xFDu; }
+    uint32_t qNanBitPattern() const { return 0xFF; }
+    uint32_t sNanBitPattern() const { return 0xFF; }
+    static constexpr lof_checker lof() {
+        return detail::make_checker(1, {0xFFu, 0xFEu, 0xFDu}, 0, 0, 0xFF, 0xFF);
+    }
+};
+template <Signedness S = Signed>
+struct IEEE_F8_NaNChecker {
+    static constexpr uint32_t code = S == Signed ? 0x80u : 0xFFu;
+    bool operator()(uint32_t bits) const { return bits == code; }
+    uint32_t qNanBitPattern() const { return code; }
+    uint32_t sNanBitPattern() const { return code; }
+    static constexpr lof_checker lof() { return detail::make_checker(1, {code}, 0, 0, code, code); }
+};
+struct IEEE_F8_InfChecker {
+    bool operator()(uint32_t bits) const { return bits == 0x7Fu || bits == 0xFFu; }
+    uint32_t minNegInf() const { return 0xFF; }
+    uint32_t minPosInf() const { return 0x7F; }
+    static constexpr lof_checker lof() {
+        return detail::make_checker(1, {0x7Fu, 0xFFu}, 0, 0, 0x7F, 0xFF);
+    }
+};
+// P3109 binary<k>: NaN is 1 << (k-1) (signed) or all-ones (unsigned); with Extended
+// behaviour, Inf is the code whose OR with 1 << (k-1) is the top code.
+template <int k, Signedness S>
+struct P_3109_NaNChecker {
+    static constexpr uint32_t code = S == Signed ? (1u << (k - 1)) : (1u << k) - 1u;
+    constexpr bool operator()(uint32_t bits) const { return bits == code; }
+    uint32_t qNanBitPattern() const { return code; }
+    uint32_t sNanBitPattern() const { return code; }
+    static constexpr lof_checker lof() { return detail::make_checker(1, {code}, 0, 0, code, code); }
+};
+template <int k, Signedness S, Inf_Behaviors H>
+struct P_3109_InfChecker {
+    static constexpr uint32_t top = S == Signed ? (1u << k) - 1u : (1u << k) - 2u;
+    constexpr bool operator()(uint32_t bits) const {
+        return H != Saturating && (bits | (1u << (k - 1))) == top;
+    }
+    uint32_t minNegInf() const { return S == Signed ? (1u << k) - 1u : 0u; }
+    uint32_t minPosInf() const { return S == Signed ? (1u << (k - 1)) - 1u : (1u << k) - 2u; }
+    static constexpr lof_checker lof() {
+        const uint32_t pos = S == Signed ? (1u << (k - 1)) - 1u : (1u << k) - 2u;
+        const uint32_t neg = S == Signed ? (1u << k) - 1u : 0u;
+        return H == Saturating ? detail::make_checker(0, {}, 0, 0, pos, neg)
+                               : detail::make_checker(3, {top}, 1u << (k - 1), 0, pos, neg);
+    }
+};
+struct OCP_E8M0_NaNChecker {
+    bool operator()(uint32_t bits) const { return bits == 0xFFu; }
+    uint32_t qNanBitPattern() const { return 0xFF; }
+    uint32_t sNanBitPattern() const { return 0xFF; }
+    static constexpr lof_checker lof() { return detail::make_checker(1, {0xFFu}, 0, 0, 0xFF, 0xFF); }
+};
+struct OCP_E8M0_InfChecker {
+    bool operator()(uint32_t) const { return false; }
+    uint32_t minNegInf() const { return 0; }
+    uint32_t minPosInf() const { return 0; }
+    static constexpr lof_checker lof() { return detail::make_checker(0, {}, 0, 0, 0, 0); }
+};
+struct FP16InfChecker {
+    bool operator()(uint32_t bits) const { return (bits & 0x7C00u) == 0x7C00u && !(bits & 0x3FFu); }
+    uint32_t minPosInf() const { return 0x7C00; }
+    uint32_t minNegInf() const { return 0xFC00; }
+    static constexpr lof_checker lof() {
+        return detail::make_checker(2, {}, 0x7C00u, 0x3FFu, 0x7C00u, 0xFC00u);
+    }
+};
+struct FP16NaNChecker {
+    bool operator()(uint32_t bits) const { return (bits & 0x7C00u) == 0x7C00u && (bits & 0x3FFu); }
+    uint32_t qNanBitPattern() const { return 0x7E00; }
+    uint32_t sNanBitPattern() const { return 0x7C01; }
+    static constexpr lof_checker lof() {
+        return detail::make_checker(2, {}, 0x7C00u, 0x3FFu, 0x7E00u, 0x7C01u);
+    }
+};
+struct BF16InfChecker {
+    bool operator()(uint32_t bits) const { return (bits & 0x7F80u) == 0x7F80u && !(bits & 0x7Fu); }
+    uint32_t minPosInf() const { return 0x7F80; }
+    uint32_t minNegInf() const { return 0xFF80; }
+    static constexpr lof_checker lof() {
+        return detail::make_checker(2, {}, 0x7F80u, 0x7Fu, 0x7F80u, 0xFF80u);
+    }
+};
+struct BF16NaNChecker {
+    bool operator()(uint32_t bits) const { return (bits & 0x7F80u) == 0x7F80u && (bits & 0x7Fu); }
+    uint32_t qNanBitPattern() const { return 0x7FC0; }
+    uint32_t sNanBitPattern() const { return 0x7F81; }
+    static constexpr lof_checker lof() {
+        return detail::make_checker(2, {}, 0x7F80u, 0x7Fu, 0x7FC0u, 0x7F81u);
+    }
+};
+
+namespace detail {
+template <class C, class = void>
+struct has_lof : std::false_type {};
+template <class C>
+struct has_lof<C, std::void_t<decltype(C::lof())>> : std::true_type {};
+
+template <class C>
+constexpr lof_checker checker_of() {
+    if constexpr (std::is_same_v<C, NoChecker>) {
+        return make_checker(0, {}, 0, 0, 0, 0);
+    } else {
+        static_assert(has_lof<C>::value,
+                      "the pack codec needs a checker with a static lof() description");
+        return C::lof();
+    }
+}
+template <class I, class N>
+inline lof_pack_format to_pack(const FloatingPointParams<I, N>& p) {
+    return lof_pack_format{to_c(p), (int32_t)p.unsigned_behavior, checker_of<I>(),
+                           checker_of<N>()};
+}
+}  // namespace detail
+
+// Bytes of a packed buffer of `count` n-bit elements (lo_pack.hpp:392-394).
+inline size_t LoF_packed_size(int n, size_t count) {
+    const int64_t v = lof_packed_size(n, (int64_t)count);
+    if (v < 0) throw Error(LOF_ERR_INVALID_ARGUMENT, "LoF_packed_size: bitwidth outside [1, 16]");
+    return (size_t)v;
+}
+
+// Round d_src[0..count) to params under ps and pack into d_dst (device pointers); returns
+// the bytes written, LoF_packed_size(params.bitwidth, count) (lo_pack.hpp:504-517).
+template <PackEndian E = PackEndian::Forward, class I, class N>
+int LoF_encode_cuda(uint8_t* d_dst, const float* d_src, FloatingPointParams<I, N> params,
+                    size_t count, ProjSpec ps = ProjSpec{}, hipStream_t stream = nullptr,
+                    const uint32_t* rand_bits = nullptr, uint64_t seed = 0) {
+    const lof_pack_format f = detail::to_pack(params);
+    detail::check(lof_pack_encode_f32(d_dst, d_src, (int64_t)count, &f, detail::to_c(ps),
+                                      E == PackEndian::Reverse ? LOF_PACK_REVERSE
+                                                               : LOF_PACK_FORWARD,
+                                      rand_bits, seed, (lof_stream_t)stream),
+                  "LoF_encode_cuda");
+    return (int)LoF_packed_size(params.bitwidth, count);
+}
+
+// Unpack `count` elements of params from d_src and widen them to float in d_dst; returns
+// the bytes consumed (lo_pack.hpp:519-531).
+template <PackEndian E = PackEndian::Forward, class I, class N>
+int LoF_decode_cuda(float* d_dst, const uint8_t* d_src, FloatingPointParams<I, N> params,
+                    size_t count, hipStream_t stream = nullptr) {
+    const lof_pack_format f = detail::to_pack(params);
+    detail::check(lof_pack_decode_f32(d_dst, d_src, (int64_t)count, &f,
+                                      E == PackEndian::Reverse ? LOF_PACK_REVERSE
+                                                               : LOF_PACK_FORWARD,
+                                      (lof_stream_t)stream),
+                  "LoF_decode_cuda");
+    return (int)LoF_packed_size(params.bitwidth, count);
+}
 
 }  // namespace lo_float

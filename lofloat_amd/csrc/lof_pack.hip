@@ -1,14 +1,20 @@
 // lof_pack.hip -- n-bit pack codec kernels and C-ABI (lof_packed_size, lof_pack_encode_f32,
 // lof_pack_decode_f32, lof_pack_reps, lof_unpack_reps).
 //
-// The packed buffer is a stream of n-bit slots (see lof_pack.hpp).  Each thread owns one
-// 32-bit word of that stream:
-//   encode: it converts the elements whose slots overlap its word (at most 32/n + 1, an
-//           element straddling two words is converted by both) and stores the word once;
-//   decode: it reads its word and the next one and produces the elements whose slots start
-//           in its word.
+// The packed buffer is a stream of n-bit slots (see lof_pack.hpp).  The group count gc is a
+// power of two (1..16) for every n in [1, 16], so slot S holds element S ^ (gc - 1) in
+// Reverse order, and a tile of kTile = 8192 elements is a whole number of groups whose
+// slots fill exactly 256 * n 32-bit words.  One 256-thread workgroup owns one tile:
+//   encode: phase 1 reads the tile's inputs as float4 (coalesced), converts each element to
+//           its rep and writes the reps into LDS (16-bit, in slot order); phase 2 packs 32
+//           slots per thread into n words, puts the tile's 256 * n words back into LDS and
+//           stores them coalesced;
+//   decode: phase 1 loads the tile's words into LDS (coalesced); phase 2 extracts four
+//           fields per thread per step from a 64-bit LDS window, widens them and stores a
+//           float4.
 // Words are accessed as 32-bit loads/stores when the buffer is 4-byte aligned and the word
-// lies inside it, else byte by byte (the buffer's byte length is groups * group_bytes).
+// lies inside it, else byte by byte (the buffer's byte length is groups * group_bytes); a
+// partial tile falls back to guarded scalar element accesses.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +28,7 @@ namespace lofamd {
 namespace {
 
 constexpr int kPackTPB = 256;
+constexpr int kTile = kPackTPB * 32;   // elements (slots) per workgroup
 
 struct PackArgs {
     const float* src;        // encode input / decode output (as dstf)
@@ -33,11 +40,12 @@ struct PackArgs {
     int64_t count;           // elements
     int64_t slots;           // groups * gc
     int64_t bytes;           // groups * group_bytes
-    int64_t words;           // ceil(bytes / 4)
+    int64_t tiles;           // ceil(count / kTile)
     const uint32_t* bits;    // random words (stochastic encode), or NULL
     uint64_t seed;
-    int32_t endian;
+    int32_t xm;              // slot of element e within the tile: e ^ xm (Reverse: gc - 1)
     int32_t aligned;         // the packed buffer is 4-byte aligned
+    int32_t vec;             // the element-side buffers (src / dstf / reps / bits) are 16-byte aligned
     PackDev P;
 };
 
@@ -48,32 +56,113 @@ __device__ __forceinline__ uint32_t pack_word_for(const PackArgs& a, int64_t e) 
     else return 0u;
 }
 
+// The rep of one element: the branch-free fast path, and float_to_rep for the lanes it
+// does not cover (a wave skips that call when none of its lanes needs it).
+template <int MODE, int RSRC, bool RAW>
+__device__ __forceinline__ uint32_t elem_rep(const PackArgs& a, uint32_t xbits, uint32_t u) {
+    if constexpr (RAW) {
+        return xbits & a.P.mask;
+    } else {
+        bool ok;
+        uint32_t rep = float_to_rep_fast<MODE>(__uint_as_float(xbits), a.P, u, ok);
+        if (!ok) rep = float_to_rep<MODE>(__uint_as_float(xbits), a.P, u);
+        return rep & a.P.mask;
+    }
+}
+
 // RAW: pack reps[] as given (masked to n bits); else quantise src[] with float_to_rep.
 template <int MODE, int RSRC, bool RAW>
 __global__ __launch_bounds__(kPackTPB) void k_pack_encode(PackArgs a) {
-    const int64_t w = (int64_t)blockIdx.x * kPackTPB + threadIdx.x;
-    if (w >= a.words) return;
-    const int n = a.P.n;
-    const int64_t b0 = w * 32;
-    const int64_t s_end = min((b0 + 31) / n, a.slots - 1);
-    uint32_t word = 0u;
-    for (int64_t S = b0 / n; S <= s_end; ++S) {
-        const int64_t e = slot_element(S, a.P.gc, a.endian);
-        uint32_t rep = 0u;   // a partial final group is padded with encodings of +0.0f: 0
-        if (e < a.count) {
-            if constexpr (RAW) rep = a.reps[e];
-            else rep = float_to_rep<MODE>(a.src[e], a.P, pack_word_for<RSRC>(a, e));
+    __shared__ __align__(16) uint16_t slot_rep[kTile];
+    const int tid = threadIdx.x;
+    const int64_t tbase = (int64_t)blockIdx.x * kTile;
+    const uint32_t* xin = RAW ? a.reps : reinterpret_cast<const uint32_t*>(a.src);
+    const bool full = a.vec && tbase + kTile <= a.count;
+    const int xq = a.xm & ~3, xi = a.xm & 3;
+    // slots of elements 4q+i are (4q ^ xq) + (i ^ xi): one aligned quad, permuted
+    auto put = [&](int q, const uint32_t (&r)[4]) {
+        uint2 packed;
+        packed.x = r[0 ^ xi] | (r[1 ^ xi] << 16);
+        packed.y = r[2 ^ xi] | (r[3 ^ xi] << 16);
+        *reinterpret_cast<uint2*>(&slot_rep[(4 * q) ^ xq]) = packed;
+    };
+    if (full) {
+        // all eight loads are issued before the first conversion waits on one
+        uint4 x[8], u[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            x[k] = reinterpret_cast<const uint4*>(xin + tbase)[k * kPackTPB + tid];
+            if constexpr (RSRC == 1) u[k] = reinterpret_cast<const uint4*>(a.bits + tbase)[k * kPackTPB + tid];
         }
-        rep &= a.P.mask;
-        const int64_t p = S * n - b0;
-        word |= p >= 0 ? (rep << p) : (rep >> (-p));
-    }
-    const int64_t byte0 = w * 4;
-    if (a.aligned && byte0 + 4 <= a.bytes) {
-        reinterpret_cast<uint32_t*>(a.dst)[w] = word;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = k * kPackTPB + tid;       // quad of elements 4q .. 4q+3 in the tile
+            if constexpr (RSRC == 2) {
+                const uint64_t e0 = (uint64_t)(tbase + 4 * q);
+                u[k] = make_uint4(rand_u32(a.seed, e0), rand_u32(a.seed, e0 + 1),
+                                  rand_u32(a.seed, e0 + 2), rand_u32(a.seed, e0 + 3));
+            } else if constexpr (RSRC == 0) {
+                u[k] = make_uint4(0u, 0u, 0u, 0u);
+            }
+            const uint32_t r[4] = {elem_rep<MODE, RSRC, RAW>(a, x[k].x, u[k].x),
+                                   elem_rep<MODE, RSRC, RAW>(a, x[k].y, u[k].y),
+                                   elem_rep<MODE, RSRC, RAW>(a, x[k].z, u[k].z),
+                                   elem_rep<MODE, RSRC, RAW>(a, x[k].w, u[k].w)};
+            put(q, r);
+        }
     } else {
-        for (int k = 0; k < 4; ++k)
-            if (byte0 + k < a.bytes) a.dst[byte0 + k] = (uint8_t)(word >> (8 * k));
+        // a partial final group is padded with encodings of +0.0f, i.e. rep 0
+        for (int k = 0; k < 8; ++k) {
+            const int q = k * kPackTPB + tid;
+            const int64_t e0 = tbase + 4 * q;
+            uint32_t r[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                r[i] = e0 + i < a.count
+                           ? elem_rep<MODE, RSRC, RAW>(a, xin[e0 + i], pack_word_for<RSRC>(a, e0 + i))
+                           : 0u;
+            put(q, r);
+        }
+    }
+    __syncthreads();
+    // Thread t packs slots 32t .. 32t+31 (whole groups: gc divides 32), i.e. words
+    // t*n .. t*n+n-1 of the tile.  The bit cursor is the same in every lane, so its branches
+    // are scalar.  The words go back into the same LDS (after a barrier), then out coalesced.
+    const int n = a.P.n;
+    uint4 v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = reinterpret_cast<const uint4*>(slot_rep)[tid * 4 + i];
+    __syncthreads();
+    uint32_t* wbuf = reinterpret_cast<uint32_t*>(slot_rep);   // 4096 words >= 256 * n
+    uint32_t cur = 0u;
+    int filled = 0, wo = tid * n;
+#pragma unroll
+    for (int sl = 0; sl < 32; ++sl) {
+        const uint4& q4 = v[sl >> 3];
+        const uint32_t d = ((sl >> 1) & 3) == 0 ? q4.x : ((sl >> 1) & 3) == 1 ? q4.y
+                         : ((sl >> 1) & 3) == 2 ? q4.z : q4.w;
+        const uint32_t rep = (sl & 1) ? (d >> 16) : (d & 0xFFFFu);
+        cur |= rep << filled;
+        filled += n;
+        if (filled >= 32) {
+            wbuf[wo++] = cur;
+            filled -= 32;
+            cur = rep >> (n - filled);   // the bits of rep past the word's end (0 if none)
+        }
+    }
+    __syncthreads();
+    const int64_t wbase = (int64_t)blockIdx.x * (kPackTPB * n);
+    for (int j = 0; j < n; ++j) {
+        const int wl = j * kPackTPB + tid;          // word of the tile
+        const int64_t byte0 = (wbase + wl) * 4;
+        if (byte0 >= a.bytes) break;
+        const uint32_t word = wbuf[wl];
+        if (a.aligned && byte0 + 4 <= a.bytes) {
+            reinterpret_cast<uint32_t*>(a.dst)[wbase + wl] = word;
+        } else {
+            for (int k = 0; k < 4; ++k)
+                if (byte0 + k < a.bytes) a.dst[byte0 + k] = (uint8_t)(word >> (8 * k));
+        }
     }
 }
 
@@ -87,22 +176,56 @@ __device__ __forceinline__ uint32_t load_word(const PackArgs& a, int64_t w) {
 }
 
 template <bool RAW>
+__device__ __forceinline__ uint32_t widen(const PackArgs& a, uint32_t rep) {
+    if constexpr (RAW) return rep;
+    else return __float_as_uint(rep_to_float(rep, a.P));
+}
+
+template <bool RAW>
 __global__ __launch_bounds__(kPackTPB) void k_pack_decode(PackArgs a) {
-    const int64_t w = (int64_t)blockIdx.x * kPackTPB + threadIdx.x;
-    if (w >= a.words) return;
+    __shared__ uint32_t words[kPackTPB * 16 + 1];
+    const int tid = threadIdx.x;
     const int n = a.P.n;
-    const int64_t b0 = w * 32;
-    const int64_t s0 = (b0 + n - 1) / n;
-    const int64_t s1 = min((b0 + 31) / n, a.slots - 1);
-    if (s0 > s1) return;
-    uint64_t win = load_word(a, w);
-    if (s1 * n + n > b0 + 32) win |= (uint64_t)load_word(a, w + 1) << 32;   // straddles
-    for (int64_t S = s0; S <= s1; ++S) {
-        const int64_t e = slot_element(S, a.P.gc, a.endian);
-        if (e >= a.count) continue;
-        const uint32_t rep = (uint32_t)(win >> (S * n - b0)) & a.P.mask;
-        if constexpr (RAW) a.repsout[e] = rep;
-        else a.dstf[e] = rep_to_float(rep, a.P);
+    const int64_t wbase = (int64_t)blockIdx.x * (kPackTPB * n);
+    if (a.aligned && (wbase + kPackTPB * n) * 4 <= a.bytes) {
+        // whole tile inside the buffer: every load issued before the first LDS write
+        uint32_t w[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j < n) w[j] = reinterpret_cast<const uint32_t*>(a.psrc)[wbase + j * kPackTPB + tid];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j < n) words[j * kPackTPB + tid] = w[j];
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const int wl = j * kPackTPB + tid;
+            words[wl] = (wbase + wl) * 4 < a.bytes ? load_word(a, wbase + wl) : 0u;
+        }
+    }
+    if (tid == 0) words[kPackTPB * n] = 0u;
+    __syncthreads();
+    const int64_t tbase = (int64_t)blockIdx.x * kTile;
+    uint32_t* out = RAW ? a.repsout : reinterpret_cast<uint32_t*>(a.dstf);
+    const bool full = a.vec && tbase + kTile <= a.count;
+    const uint32_t mask = a.P.mask;
+#pragma unroll 2
+    for (int k = 0; k < 8; ++k) {
+        const int q = k * kPackTPB + tid;
+        uint32_t v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int bit = ((4 * q + i) ^ a.xm) * n;
+            const uint64_t win = (uint64_t)words[bit >> 5] | ((uint64_t)words[(bit >> 5) + 1] << 32);
+            v[i] = widen<RAW>(a, (uint32_t)(win >> (bit & 31)) & mask);
+        }
+        const int64_t e0 = tbase + 4 * q;
+        if (full) {
+            reinterpret_cast<uint4*>(out + tbase)[q] = make_uint4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (e0 + i < a.count) out[e0 + i] = v[i];
+        }
     }
 }
 
@@ -186,29 +309,33 @@ lof_status make_pack_dev(const lof_pack_format* pf, lof_round_spec spec, PackDev
     P->is_nan = to_dev(pf->is_nan);
     P->nan_signmask = check_match(P->is_nan, P->signmask, true) ? 1 : 0;
     P->aligned_highest = (uint64_t)P->max_finite << P->r;
+    P->ah32 = (uint32_t)std::min<uint64_t>(P->aligned_highest, 0xFFFFFFFFu);
+    P->ovf_rep = (P->saturation == 0 && P->has_inf) ? P->inf_rep : P->max_finite;
+    P->fast32 = P->exp_off <= 255 ? 1 : 0;
     geometry(n, &P->gc, &P->gbits);
     return LOF_OK;
 }
 
-void size_args(PackArgs* a, int64_t count) {
+// Sizes of a call on `count` elements; false if the grid would exceed 2^31 - 1 workgroups.
+bool size_args(PackArgs* a, int64_t count, int32_t endian) {
     const int64_t groups = (count + a->P.gc - 1) / a->P.gc;
     a->count = count;
     a->slots = groups * a->P.gc;
     a->bytes = groups * (a->P.gbits / 8);
-    a->words = (a->bytes + 3) / 4;
+    a->tiles = (count + kTile - 1) / kTile;
+    a->xm = endian ? a->P.gc - 1 : 0;
+    return a->tiles < ((int64_t)1 << 31);
 }
 
-dim3 grid_for(int64_t words) { return dim3((unsigned)((words + kPackTPB - 1) / kPackTPB)); }
+dim3 grid_for(const PackArgs& a) { return dim3((unsigned)a.tiles); }
 
 template <int MODE, int RSRC>
 void launch_enc(const PackArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL((k_pack_encode<MODE, RSRC, false>), grid_for(a.words), dim3(kPackTPB), 0, st, a);
+    hipLaunchKernelGGL((k_pack_encode<MODE, RSRC, false>), grid_for(a), dim3(kPackTPB), 0, st, a);
 }
 
 lof_status launch_encode(const PackArgs& a, int mode, int rsrc, hipStream_t st) {
-    if (a.words == 0) return LOF_OK;
-    if (a.words / kPackTPB >= (int64_t)1 << 31)
-        return set_error(LOF_ERR_UNSUPPORTED, "lof_pack_encode_f32: input too large");
+    if (a.tiles == 0) return LOF_OK;
 #define LOF_PE(M, R) if (mode == M && rsrc == R) { launch_enc<M, R>(a, st); goto done; }
     LOF_PE(RNE, 0) LOF_PE(RTZ, 0) LOF_PE(RAFZ, 0) LOF_PE(RNO, 0) LOF_PE(RDOWN, 0)
     LOF_PE(RUP, 0) LOF_PE(RTA, 0) LOF_PE(RTO, 0) LOF_PE(RTTZ, 0)
@@ -252,13 +379,13 @@ LOF_API lof_status lof_pack_encode_f32(uint8_t* dst, const float* src, int64_t c
     int rsrc = 0;
     if (is_stochastic(spec.mode)) rsrc = rand_bits ? 1 : 2;
     if (uses_len(spec.mode) && spec.stoch_len == 0) rsrc = 0;   // degenerate: no draw used
-    size_args(&a, count);
+    if (!size_args(&a, count, endian)) return set_error(LOF_ERR_UNSUPPORTED, "%s: input too large", what);
     a.src = src;
     a.dst = dst;
     a.bits = rand_bits;
     a.seed = seed;
-    a.endian = endian;
     a.aligned = ((uintptr_t)dst & 3u) == 0 ? 1 : 0;
+    a.vec = aligned16(src) && (rsrc != 1 || aligned16(rand_bits)) ? 1 : 0;
     return launch_encode(a, spec.mode, rsrc, (hipStream_t)stream);
 }
 
@@ -273,13 +400,13 @@ LOF_API lof_status lof_pack_decode_f32(float* dst, const uint8_t* src, int64_t c
     if (endian != 0 && endian != 1)
         return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: endian must be 0 or 1", what);
     if (count > 0 && (!dst || !src)) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: NULL buffer", what);
-    size_args(&a, count);
-    if (a.words == 0) return LOF_OK;
+    if (!size_args(&a, count, endian)) return set_error(LOF_ERR_UNSUPPORTED, "%s: input too large", what);
+    if (a.tiles == 0) return LOF_OK;
     a.dstf = dst;
     a.psrc = src;
-    a.endian = endian;
     a.aligned = ((uintptr_t)src & 3u) == 0 ? 1 : 0;
-    hipLaunchKernelGGL((k_pack_decode<false>), grid_for(a.words), dim3(kPackTPB), 0,
+    a.vec = aligned16(dst) ? 1 : 0;
+    hipLaunchKernelGGL((k_pack_decode<false>), grid_for(a), dim3(kPackTPB), 0,
                        (hipStream_t)stream, a);
     return check_hip(hipGetLastError(), "lof_pack_decode_f32 launch");
 }
@@ -295,13 +422,13 @@ LOF_API lof_status lof_pack_reps(uint8_t* dst, const uint32_t* reps, int64_t cou
     a.P.n = bitwidth;
     a.P.mask = (1u << bitwidth) - 1u;
     geometry(bitwidth, &a.P.gc, &a.P.gbits);
-    size_args(&a, count);
-    if (a.words == 0) return LOF_OK;
+    if (!size_args(&a, count, endian)) return set_error(LOF_ERR_UNSUPPORTED, "%s: input too large", what);
+    if (a.tiles == 0) return LOF_OK;
     a.reps = reps;
     a.dst = dst;
-    a.endian = endian;
     a.aligned = ((uintptr_t)dst & 3u) == 0 ? 1 : 0;
-    hipLaunchKernelGGL((k_pack_encode<RNE, 0, true>), grid_for(a.words), dim3(kPackTPB), 0,
+    a.vec = aligned16(reps) ? 1 : 0;
+    hipLaunchKernelGGL((k_pack_encode<RNE, 0, true>), grid_for(a), dim3(kPackTPB), 0,
                        (hipStream_t)stream, a);
     return check_hip(hipGetLastError(), "lof_pack_reps launch");
 }
@@ -317,13 +444,13 @@ LOF_API lof_status lof_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t c
     a.P.n = bitwidth;
     a.P.mask = (1u << bitwidth) - 1u;
     geometry(bitwidth, &a.P.gc, &a.P.gbits);
-    size_args(&a, count);
-    if (a.words == 0) return LOF_OK;
+    if (!size_args(&a, count, endian)) return set_error(LOF_ERR_UNSUPPORTED, "%s: input too large", what);
+    if (a.tiles == 0) return LOF_OK;
     a.repsout = reps;
     a.psrc = src;
-    a.endian = endian;
     a.aligned = ((uintptr_t)src & 3u) == 0 ? 1 : 0;
-    hipLaunchKernelGGL((k_pack_decode<true>), grid_for(a.words), dim3(kPackTPB), 0,
+    a.vec = aligned16(reps) ? 1 : 0;
+    hipLaunchKernelGGL((k_pack_decode<true>), grid_for(a), dim3(kPackTPB), 0,
                        (hipStream_t)stream, a);
     return check_hip(hipGetLastError(), "lof_unpack_reps launch");
 }

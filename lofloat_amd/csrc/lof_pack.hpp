@@ -53,6 +53,9 @@ struct PackDev {
     uint32_t signmask, max_finite, inf_rep, qnan, snan, mask;
     int32_t nan_signmask;    // IsNaN(signmask): no -0 code
     uint64_t aligned_highest;
+    uint32_t ah32;           // min(aligned_highest, 2^32 - 1)
+    uint32_t ovf_rep;        // magnitude of an overflowing value: saturation 0 with Inf -> Inf
+    int32_t fast32;          // exp_off <= 255: the normal path fits 32-bit arithmetic
     CheckDev is_inf, is_nan; // decode
     int32_t gc, gbits;       // group geometry
 };
@@ -181,8 +184,71 @@ __device__ __forceinline__ uint32_t float_to_rep(float from, const PackDev& P, u
     return with_sign((uint32_t)(rb >> P.r));
 }
 
-// convert_rep_to_float (lo_pack.hpp:268-304): Inf checked before NaN, the value computed
-// in double from the fields, then rounded to float.
+// round_bits64 for operands where bits + 2^r < 2^32 (r in [1, 24]): the same integers in
+// 32-bit arithmetic.
+template <int MODE>
+__device__ __forceinline__ uint32_t round_bits32(uint32_t bits, int r, int len, uint32_t u) {
+    const uint32_t one = 1u << r, low = one - 1u;
+    const uint32_t T = bits & ~low, tail = bits & low;
+    const uint32_t R = len > 0 ? (u & (uint32_t)(((uint64_t)1 << len) - 1u)) : 0u;
+    if constexpr (MODE == RNE) return bits + ((bits >> r) & 1u) + (one >> 1) - 1u;
+    else if constexpr (MODE == RTZ) return T;
+    else if constexpr (MODE == RAFZ) return T + (tail ? one : 0u);
+    else if constexpr (MODE == SRA) return len <= 0 ? bits : bits + (R << (r - len));
+    else if constexpr (MODE == RNO) return bits + ((~bits >> r) & 1u) + (one >> 1) - 1u;
+    else if constexpr (MODE == RDOWN) return T + (tail ? one : 0u);   // pos = false
+    else if constexpr (MODE == RUP) return T;                          // pos = false
+    else if constexpr (MODE == RTA) return T + (((bits >> (r - 1)) & 1u) << r);
+    else if constexpr (MODE == SRB) return bits + (tail > 0 ? ((R + 1u) << (r - len)) : 0u);
+    else if constexpr (MODE == SRC)
+        return (len == 0 || tail == 0) ? bits : bits + (((R >> 1) + (R & 1u)) << (r - len + 1));
+    else if constexpr (MODE == TRUESR)   // u * 2^r < tail * 2^32  <=>  u < tail * 2^(32-r)
+        return (uint64_t)u < ((uint64_t)tail << (32 - r)) ? T + one : T;
+    else if constexpr (MODE == PROB) return T + (tail ? ((u & 1u) << r) : 0u);
+    else if constexpr (MODE == RTO) return T | (tail ? one : 0u);
+    else if constexpr (MODE == RTTZ) return bits + (one >> 1) - 1u;
+    else if constexpr (MODE == SRD)
+        return len <= 0 ? bits : bits + ((R + ((u >> len) & 1u)) << (r - len));
+    else return bits;
+}
+
+// float_to_rep without branches for the inputs that need no special handling: finite,
+// not negative into an unsigned format, and either zero, a normal FP32 value on the normal
+// path, or (bias < 126) a value below the format's normal range whose alignment shift is
+// <= 24.
+// Sets `ok` false for every other input, whose rep the caller takes from float_to_rep; where
+// ok is true the result equals float_to_rep's (the same integers, in 32-bit arithmetic: the
+// exponent rebias cannot wrap for these inputs, and P.fast32 bounds it above).
+template <int MODE>
+__device__ __forceinline__ uint32_t float_to_rep_fast(float from, const PackDev& P, uint32_t u,
+                                                      bool& ok) {
+    const uint32_t xb = __float_as_uint(from);
+    const bool neg = (xb >> 31) != 0u;
+    const uint32_t fb = xb & 0x7FFFFFFFu;
+    const int bfe = (int)(fb >> 23);
+    const int bte = bfe - 127 + P.bias;
+    const bool sub = P.branch == 2 && bte <= 0;
+    const uint32_t lead = bfe > 0 ? 1u : 0u;
+    const int sh = min(P.r - bte + (int)lead, 31);
+    // normal path (lo_pack.hpp:242-257)
+    uint32_t rb = round_bits32<MODE>(fb, P.r, P.stoch_len, u) & ~((1u << P.r) - 1u);
+    rb += (uint32_t)P.exp_off << 23;
+    const bool ovf = P.narrower && rb > P.ah32;
+    uint32_t mag = ovf ? P.ovf_rep : (rb >> P.r);
+    if (sub) {   // below the normal range (lo_pack.hpp:201-239), shift <= 24
+        const uint32_t rfb = (fb & 0x7FFFFFu) | (lead << 23);
+        mag = round_bits32<MODE>(rfb, sh, P.stoch_len, u) >> sh;
+    }
+    mag = fb == 0u ? 0u : mag;
+    ok = P.fast32 && fb < 0x7F800000u && !(neg && !P.is_signed) &&
+         (fb == 0u || (sub ? sh <= 24 : bfe > 0));
+    const bool sgn = neg && P.is_signed;
+    return !sgn ? mag : ((mag == 0u && P.nan_signmask) ? 0u : (mag | P.signmask));
+}
+
+// convert_rep_to_float (lo_pack.hpp:268-304): Inf checked before NaN, the value formed in
+// double from the fields, then rounded to float.  (1 + m / 2^mb) * 2^(e - bias) is formed
+// as (2^mb + m) * 2^(e - bias - mb): the same double, without a division.
 __device__ __forceinline__ float rep_to_float(uint32_t rep, const PackDev& P) {
     rep &= P.mask;
     const bool neg = P.is_signed && ((rep >> (P.n - 1)) & 1u);
@@ -191,9 +257,10 @@ __device__ __forceinline__ float rep_to_float(uint32_t rep, const PackDev& P) {
         return neg ? -__builtin_huge_valf() : __builtin_huge_valf();
     if (check_match(P.is_nan, rep, true)) return __uint_as_float(0x7FC00000u);
     const uint32_t e = mag >> P.mb, m = mag & ((1u << P.mb) - 1u);
-    const double v = e == 0u ? ldexp((double)m, 1 - P.bias - P.mb)
-                             : ldexp(1.0 + (double)m / (double)(1u << P.mb), (int)e - P.bias);
-    return (float)(neg ? -v : v);
+    const uint32_t sig = e == 0u ? m : (m | (1u << P.mb));
+    const int k = (e == 0u ? 1 : (int)e) - P.bias - P.mb;
+    const float v = (float)ldexp((double)sig, k);
+    return neg ? -v : v;
 }
 
 // The element held by slot S of the packed stream (S-th n-bit field).

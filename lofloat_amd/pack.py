@@ -1,6 +1,6 @@
 """Bit-pack codec on the device: quantise FP32 tensors to n-bit reps (1 <= n <= 16) and pack
 them tightly, or unpack and widen them back (reference lo_pack.hpp: LoF_encode / LoF_decode /
-LoF_packed_size / LoF_encode_cuda / LoF_decode_cuda, lo_pack.hpp:398-530).
+LoF_packed_size / LoF_encode_cuda / LoF_decode_cuda, lo_pack.hpp:392-531).
 
 A group of lcm(n, 8) bits (n <= 8) or lcm(n, 16) bits holds bits / n elements; fields are
 LSB-first within bytes; ``endian="forward"`` puts element i of a group in slot i,

@@ -124,16 +124,16 @@ typedef struct orc_pack_format {
 uint32_t orc_float_to_rep(float x, const orc_pack_format* f, orc_spec ps, uint32_t u);
 /* convert_rep_to_float (lo_pack.hpp:268-304). */
 float orc_rep_to_float(uint32_t rep, const orc_pack_format* f);
-/* LoF_packed_size (lo_pack.hpp:398-402). */
+/* LoF_packed_size (lo_pack.hpp:392-394). */
 int64_t orc_packed_bytes(int n, int64_t count);
-/* LoF_encode(float) / LoF_decode(float) (lo_pack.hpp:421-467); bits == NULL -> words
+/* LoF_encode(float) / LoF_decode(float) (lo_pack.hpp:414-463); bits == NULL -> words
  * orc_rand_u32(seed, i) for element i.  Return bytes written / consumed. */
 int64_t orc_pack_encode(uint8_t* dst, const float* src, int64_t count,
                         const orc_pack_format* f, orc_spec ps, int endian,
                         const uint32_t* bits, uint64_t seed);
 int64_t orc_pack_decode(float* dst, const uint8_t* src, int64_t count,
                         const orc_pack_format* f, int endian);
-/* LoF_encode / LoF_decode of typed values (lo_pack.hpp:405-418, 441-454): raw reps. */
+/* LoF_encode / LoF_decode of typed values (lo_pack.hpp:397-412, 431-446): raw reps. */
 int64_t orc_pack_reps(uint8_t* dst, const uint32_t* reps, int64_t count, int n, int endian);
 int64_t orc_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t count, int n, int endian);
 

@@ -48,6 +48,69 @@ static int compare(const std::vector<float>& got, const std::vector<float>& want
     return (int)bad;
 }
 
+static orc_pack_format to_orc(const lof_pack_format& p) {
+    static_assert(sizeof(orc_pack_format) == sizeof(lof_pack_format), "same layout");
+    orc_pack_format o;
+    std::memcpy(&o, &p, sizeof o);
+    return o;
+}
+
+// The lof() description of each checker, as the oracle's decoder reads it, classifies every
+// code of the format as the functor's own operator() does.
+template <class I, class N>
+static void checker_case(const char* name, FloatingPointParams<I, N> p) {
+    const orc_pack_format of = to_orc(detail::to_pack(p));
+    const uint32_t codes = 1u << p.bitwidth;
+    std::vector<uint32_t> reps(codes);
+    for (uint32_t r = 0; r < codes; ++r) reps[r] = r;
+    std::vector<float> v(codes);
+    orc_rep_to_float_array(reps.data(), v.data(), codes, &of);
+    int bad = 0;
+    for (uint32_t r = 0; r < codes; ++r) {
+        const bool inf = p.OV_behavior != Saturating && p.IsInf(r);
+        const bool nan = !inf && p.IsNaN(r);
+        bad += std::isinf(v[r]) != inf || std::isnan(v[r]) != nan;
+    }
+    EXPECT(bad == 0, "checker description %s: %d codes classified differently", name, bad);
+}
+
+static void pack_cpu_checks() {
+    EXPECT(LoF_packed_size(11, 17) == 44 && LoF_packed_size(3, 9) == 6 &&
+               LoF_packed_size(16, 5) == 10 && LoF_packed_size(8, 0) == 0,
+           "LoF_packed_size");
+    try {
+        (void)LoF_packed_size(17, 1);
+        EXPECT(false, "bitwidth 17 must throw");
+    } catch (const Error& e) {
+        EXPECT(e.status == LOF_ERR_INVALID_ARGUMENT, "status for bitwidth 17");
+    }
+    checker_case("e4m3fn", FloatingPointParams(8, 3, 7, Saturating, _3109, Signed,
+                                               OCP_F8E4M3_InfChecker{}, OCP_F8E4M3_NaNChecker{}));
+    checker_case("e5m2", FloatingPointParams(8, 2, 15, Saturating, _3109, Signed,
+                                             OCP_F8E4M3_InfChecker{}, OCP_F8E5M2_NaNChecker{}));
+    checker_case("ieee_f8", FloatingPointParams(8, 3, 7, Extended, _3109, Signed,
+                                                IEEE_F8_InfChecker{}, IEEE_F8_NaNChecker<Signed>{}));
+    checker_case("ieee_f8u", FloatingPointParams(8, 3, 7, Extended, _3109, Unsigned,
+                                                 IEEE_F8_InfChecker{}, IEEE_F8_NaNChecker<Unsigned>{}));
+    checker_case("p3109<8,4>ext", FloatingPointParams(8, 3, 8, Extended, _3109, Signed,
+                                                      P_3109_InfChecker<8, Signed, Extended>{},
+                                                      P_3109_NaNChecker<8, Signed>{}));
+    checker_case("p3109<6,3>u ext", FloatingPointParams(6, 2, 4, Extended, _3109, Unsigned,
+                                                        P_3109_InfChecker<6, Unsigned, Extended>{},
+                                                        P_3109_NaNChecker<6, Unsigned>{}));
+    checker_case("p3109<11,5>", FloatingPointParams(11, 4, 32, Saturating, _3109, Signed,
+                                                    P_3109_InfChecker<11, Signed, Saturating>{},
+                                                    P_3109_NaNChecker<11, Signed>{}));
+    checker_case("e8m0", FloatingPointParams(8, 0, 127, Saturating, _3109, Unsigned,
+                                             OCP_E8M0_InfChecker{}, OCP_E8M0_NaNChecker{}));
+    checker_case("half", FloatingPointParams(16, 10, 15, Extended, _3109, Signed, FP16InfChecker{},
+                                             FP16NaNChecker{}));
+    checker_case("bf16", FloatingPointParams(16, 7, 127, Extended, _3109, Signed, BF16InfChecker{},
+                                             BF16NaNChecker{}));
+    checker_case("trivial", FloatingPointParams(5, 2, 3, Saturating, _3109, Signed,
+                                                Trivial_InfChecker{}, Trivial_NaNChecker{}));
+}
+
 static void cpu_checks() {
     ProjSpec ps;
     EXPECT(ps.rounding_mode == Rounding_Mode::RoundToNearestEven && ps.stoch_length == 0,
@@ -68,6 +131,7 @@ static void cpu_checks() {
     EXPECT(g.can_implement(A, B, C, D) == GemmR::Status::kErrorNotSupported,
            "shape mismatch rejected");
     EXPECT(g.get_workspace_size(A, B, C, D) == 0, "no workspace");
+    pack_cpu_checks();
 }
 
 template <class T>
@@ -140,6 +204,66 @@ static void gemm_case(std::mt19937& rng, float beta) {
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dD);
 }
 
+// LoF_encode_cuda / LoF_decode_cuda against orc_pack_encode / orc_pack_decode: bytes and
+// decoded values bit-exact.
+template <PackEndian E, class I, class N>
+static void pack_case(const char* name, FloatingPointParams<I, N> p, const std::vector<float>& x,
+                      ProjSpec ps = ProjSpec{}, uint64_t seed = 0) {
+    const size_t n = x.size(), nb = LoF_packed_size(p.bitwidth, n);
+    float* dx = dev_copy(x);
+    uint8_t* db = dev_copy(std::vector<uint8_t>(nb + 1, 0xA5));
+    const int wrote = LoF_encode_cuda<E>(db, dx, p, n, ps, nullptr, nullptr, seed);
+    float* dy = dev_copy(std::vector<float>(n, 0.f));
+    const int read = LoF_decode_cuda<E>(dy, db, p, n);
+    (void)hipDeviceSynchronize();
+    EXPECT(wrote == (int)nb && read == (int)nb, "%s: byte counts %d %d vs %zu", name, wrote, read, nb);
+    const orc_pack_format of = to_orc(detail::to_pack(p));
+    const int endian = E == PackEndian::Reverse ? 1 : 0;
+    std::vector<uint8_t> want(nb + 1, 0xA5);
+    orc_pack_encode(want.data(), x.data(), (int64_t)n, &of,
+                    orc_spec{(int32_t)ps.rounding_mode, (int32_t)ps.saturation_mode, ps.stoch_length},
+                    endian, nullptr, seed);
+    const std::vector<uint8_t> got = host_copy(db, nb + 1);
+    size_t bad = 0;
+    for (size_t i = 0; i <= nb; ++i) bad += got[i] != want[i];
+    EXPECT(bad == 0, "%s: %zu packed bytes differ (incl. the guard byte)", name, bad);
+    std::vector<float> dec(n);
+    orc_pack_decode(dec.data(), want.data(), (int64_t)n, &of, endian);
+    char what[96];
+    std::snprintf(what, sizeof(what), "%s decode", name);
+    compare(host_copy(dy, n), dec, what);
+    (void)hipFree(dx); (void)hipFree(db); (void)hipFree(dy);
+}
+
+static void pack_gpu_checks(std::mt19937& rng) {
+    std::vector<float> x(50001);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    std::uniform_real_distribution<float> ud(-20.f, 20.f);
+    for (auto& v : x) v = nd(rng) * std::exp2(ud(rng));
+    x[0] = 0.f; x[1] = -0.f; x[2] = INFINITY; x[3] = -INFINITY; x[4] = NAN; x[5] = 1e-40f;
+    const FloatingPointParams p11(11, 4, 32, Saturating, _3109, Signed,
+                                  P_3109_InfChecker<11, Signed, Saturating>{},
+                                  P_3109_NaNChecker<11, Signed>{});
+    pack_case<PackEndian::Forward>("p3109<11,5> forward", p11, x);
+    pack_case<PackEndian::Reverse>("p3109<11,5> reverse", p11, x);
+    pack_case<PackEndian::Reverse>("e5m2 reverse", FloatingPointParams(
+        8, 2, 15, Saturating, _3109, Signed, OCP_F8E4M3_InfChecker{}, OCP_F8E5M2_NaNChecker{}), x);
+    pack_case<PackEndian::Forward>("half SatFinite", FloatingPointParams(
+        16, 10, 15, Extended, _3109, Signed, FP16InfChecker{}, FP16NaNChecker{}), x,
+        ProjSpec(Rounding_Mode::RoundToNearestEven, SatFinite));
+    pack_case<PackEndian::Forward>("bf16 RTZ", FloatingPointParams(
+        16, 7, 127, Extended, _3109, Signed, BF16InfChecker{}, BF16NaNChecker{}), x,
+        ProjSpec(Rounding_Mode::RoundTowardsZero));
+    pack_case<PackEndian::Reverse>("ieee_f8 SR-B", FloatingPointParams(
+        8, 3, 7, Extended, _3109, Signed, IEEE_F8_InfChecker{}, IEEE_F8_NaNChecker<Signed>{}), x,
+        ProjSpec(Rounding_Mode::StochasticRoundingB, OvfInf, 6), 77);
+    pack_case<PackEndian::Forward>("e2m1 (no checkers)",
+                                   FloatingPointParams<>(4, 1, 1, Saturating, _3109, Signed), x);
+    pack_case<PackEndian::Forward>("p3109<6,3>u NegtoNaN", FloatingPointParams(
+        6, 2, 4, Saturating, _3109, Unsigned, P_3109_InfChecker<6, Unsigned, Saturating>{},
+        P_3109_NaNChecker<6, Unsigned>{}, NegtoNaN), x);
+}
+
 static void gpu_checks() {
     std::mt19937 rng(1234);
     const int n = 100003;
@@ -183,6 +307,7 @@ static void gpu_checks() {
     compare(host_copy(dm, n), want, "run_virtual_mx_round e2m1");
     (void)hipFree(d); (void)hipFree(dw); (void)hipFree(dout); (void)hipFree(din); (void)hipFree(dm);
 
+    pack_gpu_checks(rng);
     for (float beta : {0.0f, 0.75f}) {
         gemm_case<RowMajor, RowMajor, RowMajor>(rng, beta);
         gemm_case<ColMajor, RowMajor, RowMajor>(rng, beta);

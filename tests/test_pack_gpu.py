@@ -39,12 +39,17 @@ def _descs():
         "p3109<8,4>ext": p3109(8, 4, True, True), "u_p3109<6,3>": p3109(6, 3, False),
         "ieee_f8": lof.FloatFormatDescriptor(8, 3, 7, I.Extended, N._3109, S.Signed,
                                              F.IEEEF8InfChecker(), F.IEEEF8NaNChecker(True)),
+        # 10 exponent bits, bias 150: values from 2^-154 to 2^873, beyond FP32 both ways
+        "wide16": lof.FloatFormatDescriptor(
+            16, 5, 150, I.Extended, N._3109, S.Signed,
+            F.MaskedInfChecker(0x7FE0, 0x1F, 0x7FE0, 0xFFE0),
+            F.MaskedNaNChecker(0x7FE0, 0x1F, 0x7FF0, 0x7FE1)),
     }
 
 
 NAMES = ["p3109<3,2>", "e2m1", "p3109<5,2>", "e3m2", "p3109<7,3>", "e5m2", "e4m3fn",
          "p3109<9,3>", "p3109<11,5>", "p3109<12,8>", "half", "bf16", "p3109<8,4>ext",
-         "u_p3109<6,3>", "ieee_f8"]
+         "u_p3109<6,3>", "ieee_f8", "wide16"]
 
 
 def _opf(desc, ub=0):
@@ -137,7 +142,7 @@ def test_unsigned_behaviour(dev, ub):
 def test_pack_reps_roundtrip(dev, n, endian):
     import lofloat_amd as lof
     g = np.random.default_rng(n * 3 + endian)
-    for count in (1, 7, 33, 1000, 4099):
+    for count in (1, 7, 33, 1000, 4099, 3 * 8192 + 5):
         reps = g.integers(0, 1 << 31, count, dtype=np.int64).astype(np.int32)
         rt = torch.from_numpy(reps).to(dev)
         buf = lof.pack.pack_reps(rt, n, endian)
@@ -157,16 +162,27 @@ def test_sizes_and_unaligned(dev):
         got = _gpu_encode(lof, x, desc, dev)
         assert got.size == lof.pack.packed_size(11, count)
         assert np.array_equal(got, oracle.pack_encode(x, pf)), count
-    # unaligned packed buffers (byte offset 1) give the same bytes and values
-    x = g.standard_normal(1001).astype(np.float32)
-    nb = lof.pack.packed_size(11, x.size)
-    big = torch.zeros(nb + 5, dtype=torch.uint8, device=dev)
-    xt = torch.from_numpy(x).to(dev)
-    lof.pack.encode(xt, desc, out=big[1:1 + nb])
-    aligned = lof.pack.encode(xt, desc)
-    assert torch.equal(big[1:1 + nb], aligned)
-    d1 = lof.pack.decode(big[1:1 + nb], desc, x.size)
-    assert torch.equal(d1, lof.pack.decode(aligned, desc, x.size))
+    # unaligned packed buffers (byte offset 1) give the same bytes and values, and nothing
+    # is written past the packed length; so do float buffers that are not 16-byte aligned
+    for count in (1001, 5 * 8192 + 77):
+        x = g.standard_normal(count).astype(np.float32)
+        nb = lof.pack.packed_size(11, x.size)
+        big = torch.full((nb + 5,), 0xA5, dtype=torch.uint8, device=dev)
+        xt = torch.from_numpy(x).to(dev)
+        lof.pack.encode(xt, desc, out=big[1:1 + nb])
+        aligned = lof.pack.encode(xt, desc)
+        assert torch.equal(big[1:1 + nb], aligned), count
+        assert big[0].item() == 0xA5 and bool((big[1 + nb:] == 0xA5).all()), count
+        assert np.array_equal(aligned.cpu().numpy(), oracle.pack_encode(x, pf)), count
+        d1 = lof.pack.decode(big[1:1 + nb], desc, x.size)
+        d0 = lof.pack.decode(aligned, desc, x.size)
+        assert torch.equal(d1, d0), count
+        xs = torch.zeros(count + 1, device=dev)
+        xs[1:] = xt                                  # 4-byte but not 16-byte aligned
+        assert torch.equal(lof.pack.encode(xs[1:], desc), aligned), count
+        ys = torch.zeros(count + 1, device=dev)
+        lof.pack.decode(aligned, desc, count, out=ys[1:])
+        assert torch.equal(ys[1:], d0), count
 
 
 def test_large_e4m3fn(dev):
