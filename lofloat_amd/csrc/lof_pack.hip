@@ -309,7 +309,7 @@ lof_status make_pack_dev(const lof_pack_format* pf, lof_round_spec spec, PackDev
     P->is_nan = to_dev(pf->is_nan);
     P->nan_signmask = check_match(P->is_nan, P->signmask, true) ? 1 : 0;
     P->aligned_highest = (uint64_t)P->max_finite << P->r;
-    P->ah32 = (uint32_t)std::min<uint64_t>(P->aligned_highest, 0xFFFFFFFFu);
+    P->off2 = (uint32_t)((int64_t)P->exp_off * ((int64_t)1 << mb));
     P->ovf_rep = (P->saturation == 0 && P->has_inf) ? P->inf_rep : P->max_finite;
     P->fast32 = P->exp_off <= 255 ? 1 : 0;
     geometry(n, &P->gc, &P->gbits);

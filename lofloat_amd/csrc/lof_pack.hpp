@@ -53,7 +53,7 @@ struct PackDev {
     uint32_t signmask, max_finite, inf_rep, qnan, snan, mask;
     int32_t nan_signmask;    // IsNaN(signmask): no -0 code
     uint64_t aligned_highest;
-    uint32_t ah32;           // min(aligned_highest, 2^32 - 1)
+    uint32_t off2;           // exp_off * 2^mb (mod 2^32): the rebias after the shift
     uint32_t ovf_rep;        // magnitude of an overflowing value: saturation 0 with Inf -> Inf
     int32_t fast32;          // exp_off <= 255: the normal path fits 32-bit arithmetic
     CheckDev is_inf, is_nan; // decode
@@ -214,11 +214,17 @@ __device__ __forceinline__ uint32_t round_bits32(uint32_t bits, int r, int len, 
 
 // float_to_rep without branches for the inputs that need no special handling: finite,
 // not negative into an unsigned format, and either zero, a normal FP32 value on the normal
-// path, or (bias < 126) a value below the format's normal range whose alignment shift is
-// <= 24.
-// Sets `ok` false for every other input, whose rep the caller takes from float_to_rep; where
-// ok is true the result equals float_to_rep's (the same integers, in 32-bit arithmetic: the
-// exponent rebias cannot wrap for these inputs, and P.fast32 bounds it above).
+// path, or (bias < 126) a value below the format's normal range (for the stochastic modes,
+// only where its alignment shift is <= 24).  Sets `ok` false for every other input, whose rep the caller takes from
+// float_to_rep; where ok is true the result equals float_to_rep's.
+//   normal path (lo_pack.hpp:242-257): ((round(fb, r) & ~low) + exp_off * 2^23) >> r
+//       = (round(fb, r) >> r) + exp_off * 2^mb, exact in 32 bits here (the rebias cannot
+//       wrap for these inputs, P.fast32 bounds it above), and its overflow test
+//       rb > max_finite * 2^r is the same as (rb >> r) > max_finite;
+//   below the normal range (lo_pack.hpp:201-239): round(rfb, shift) >> shift, taken only by
+//       the lanes that need it (its shift is per lane; the normal path's is uniform); for
+//       shift > 24 (rfb < 2^24 <= half the unit) the deterministic modes give 0, except the
+//       directed ones, which the reference evaluates with the sign.
 template <int MODE>
 __device__ __forceinline__ uint32_t float_to_rep_fast(float from, const PackDev& P, uint32_t u,
                                                       bool& ok) {
@@ -226,22 +232,27 @@ __device__ __forceinline__ uint32_t float_to_rep_fast(float from, const PackDev&
     const bool neg = (xb >> 31) != 0u;
     const uint32_t fb = xb & 0x7FFFFFFFu;
     const int bfe = (int)(fb >> 23);
-    const int bte = bfe - 127 + P.bias;
+    const int bte = bfe + P.exp_off;              // bfe - 127 + bias
     const bool sub = P.branch == 2 && bte <= 0;
     const uint32_t lead = bfe > 0 ? 1u : 0u;
-    const int sh = min(P.r - bte + (int)lead, 31);
-    // normal path (lo_pack.hpp:242-257)
-    uint32_t rb = round_bits32<MODE>(fb, P.r, P.stoch_len, u) & ~((1u << P.r) - 1u);
-    rb += (uint32_t)P.exp_off << 23;
-    const bool ovf = P.narrower && rb > P.ah32;
-    uint32_t mag = ovf ? P.ovf_rep : (rb >> P.r);
-    if (sub) {   // below the normal range (lo_pack.hpp:201-239), shift <= 24
-        const uint32_t rfb = (fb & 0x7FFFFFu) | (lead << 23);
-        mag = round_bits32<MODE>(rfb, sh, P.stoch_len, u) >> sh;
+    const int shs = min(P.r - bte + (int)lead, 31);
+    uint32_t mag = (round_bits32<MODE>(fb, P.r, P.stoch_len, u) >> P.r) + P.off2;
+    mag = (P.narrower && mag > P.max_finite) ? P.ovf_rep : mag;
+    if (sub) {
+        const int s = min(shs, 24);
+        mag = round_bits32<MODE>((fb & 0x7FFFFFu) | (lead << 23), s, P.stoch_len, u) >> s;
+        if constexpr (!is_stochastic(MODE)) {
+            // shift > 24 with a nonzero significand below 2^24: only the directed modes
+            // give a 1 (lo_pack.hpp:214-222); every other deterministic mode gives 0
+            const uint32_t deep = (MODE == RTO || MODE == RAFZ) ? 1u
+                                  : MODE == RUP ? (neg ? 0u : 1u)
+                                  : MODE == RDOWN ? (neg ? 1u : 0u) : 0u;
+            mag = shs > 24 ? deep : mag;
+        }
     }
     mag = fb == 0u ? 0u : mag;
     ok = P.fast32 && fb < 0x7F800000u && !(neg && !P.is_signed) &&
-         (fb == 0u || (sub ? sh <= 24 : bfe > 0));
+         (fb == 0u || (sub ? (!is_stochastic(MODE) || shs <= 24) : bfe > 0));
     const bool sgn = neg && P.is_signed;
     return !sgn ? mag : ((mag == 0u && P.nan_signmask) ? 0u : (mag | P.signmask));
 }
@@ -253,14 +264,14 @@ __device__ __forceinline__ float rep_to_float(uint32_t rep, const PackDev& P) {
     rep &= P.mask;
     const bool neg = P.is_signed && ((rep >> (P.n - 1)) & 1u);
     const uint32_t mag = P.is_signed ? (rep & (P.signmask - 1u)) : rep;
-    if (P.has_inf && check_match(P.is_inf, rep, false))
-        return neg ? -__builtin_huge_valf() : __builtin_huge_valf();
-    if (check_match(P.is_nan, rep, true)) return __uint_as_float(0x7FC00000u);
+    const bool inf = P.has_inf && check_match(P.is_inf, rep, false);
+    const bool nan = check_match(P.is_nan, rep, true);
     const uint32_t e = mag >> P.mb, m = mag & ((1u << P.mb) - 1u);
     const uint32_t sig = e == 0u ? m : (m | (1u << P.mb));
     const int k = (e == 0u ? 1 : (int)e) - P.bias - P.mb;
-    const float v = (float)ldexp((double)sig, k);
-    return neg ? -v : v;
+    uint32_t vb = __float_as_uint((float)ldexp((double)sig, k));
+    vb = inf ? 0x7F800000u : (nan ? 0x7FC00000u : vb);
+    return __uint_as_float(vb | ((neg && !nan) || (neg && inf) ? 0x80000000u : 0u));
 }
 
 // The element held by slot S of the packed stream (S-th n-bit field).

@@ -263,3 +263,90 @@ def test_stochastic_encode_with_words():
     assert np.array_equal(a, b)
     up = np.mean(oracle.float_to_rep(x, pf, oracle.SRA, stoch_len=8, bits=w) == 0x39)
     assert abs(up - 0.375) < 0.01
+
+
+@pytest.mark.parametrize("name,fmt,preset", FORMATS[:12], ids=[f[0] for f in FORMATS[:12]])
+def test_stochastic_encode_equals_virtual_round(name, fmt, preset):
+    """In the format's normal range the codec's stochastic rounding is the pinned
+    virtual_round oracle's, word for word, in all six stochastic modes.  (Below the normal
+    range the two reference functions align the significand differently; that range is
+    pinned by test_stochastic_encode_closed_form.)"""
+    pf = _pf(fmt, preset)
+    reps = np.arange(1 << fmt[0], dtype=np.uint32)
+    fin = oracle.rep_to_float(reps, pf)
+    fin = fin[np.isfinite(fin)]
+    top = float(np.max(np.abs(fin)))
+    g = np.random.default_rng(fmt[0] + 100)
+    x = (g.uniform(-1, 1, 60000) * top).astype(np.float32)
+    x = np.concatenate([x, _probe_values(fmt, pf)])
+    x = x[np.isfinite(x) & (np.abs(x) <= top) & (np.abs(x) >= max(2.0 ** (1 - fmt[2]), 2.0 ** -126))]
+    for mode in oracle.STOCHASTIC_MODES:
+        L = 0 if mode in (oracle.TRUESR, oracle.PROB) else min(5, 23 - fmt[1])
+        w = oracle.rand_u32(7 + mode, x.size)
+        dec = oracle.rep_to_float(oracle.float_to_rep(x, pf, mode, stoch_len=L, bits=w), pf)
+        vr = oracle.virtual_round(x, fmt, mode, L, bits=w)
+        on_grid = np.isin(np.abs(vr), np.abs(fin))   # overflow passes through unrounded
+        ok = _bits_eq(dec, vr) | np.isnan(dec) | ~on_grid
+        assert ok.all(), (name, mode, x[~ok][:5], dec[~ok][:5], vr[~ok][:5])
+        assert on_grid.mean() > 0.9
+
+
+def _stochastic_up(mode, f, L, u):
+    """Whether a stochastic mode rounds a magnitude with fractional part f (0 <= f < 1, in
+    units of the target's last place) up, given the element's 32-bit word u: the mode
+    definitions (R = u mod 2^L; SR-D adds bit L of u; TrueSR compares u with f * 2^32;
+    Probabilistic takes u's low bit)."""
+    R = u & ((1 << L) - 1) if L > 0 else 0
+    if mode == oracle.SRA:
+        return L > 0 and math.floor(f * 2 ** L) + R >= 2 ** L
+    if mode == oracle.SRB:
+        return f > 0 and math.floor(f * 2 ** L) + R + 1 >= 2 ** L
+    if mode == oracle.SRC:
+        return L > 0 and f > 0 and math.floor(f * 2 ** (L - 1)) + (R >> 1) + (R & 1) >= 2 ** (L - 1)
+    if mode == oracle.SRD:
+        return L > 0 and math.floor(f * 2 ** L) + R + ((u >> L) & 1) >= 2 ** L
+    if mode == oracle.TRUESR:
+        return u < f * 2 ** 32
+    if mode == oracle.PROB:
+        return f > 0 and (u & 1) == 1
+    raise ValueError(mode)
+
+
+@pytest.mark.parametrize("name,fmt,preset", [FORMATS[i] for i in (2, 5, 6, 8, 10)],
+                         ids=[FORMATS[i][0] for i in (2, 5, 6, 8, 10)])
+def test_stochastic_encode_closed_form(name, fmt, preset):
+    """Below the format's normal range (bias < 126, so these are the reference's
+    'fewer exponents near zero' branch), including FP32-subnormal inputs and values far
+    below the smallest subnormal: the encoded value is sign * (floor(q) + up) * 2^(1-bias-mb)
+    with q = |x| / 2^(1-bias-mb) in exact rationals and `up` from the mode's definition
+    (SR-D's coin can add a whole unit to an exact multiple, as in virtual_round)."""
+    pf = _pf(fmt, preset)
+    n, mb, bias = fmt[0], fmt[1], fmt[2]
+    unit = Fraction(1, 2 ** (bias - 1 + mb))
+    g = np.random.default_rng(n * 13 + mb)
+    mags = np.exp2(g.uniform(-152, 1 - bias, 1500)).astype(np.float32)
+    mags = np.concatenate([mags, np.float32(unit) * np.array([0.5, 0.25, 1.5, 3.0], np.float32),
+                           np.array([1e-45, 3e-44, 1e-40], np.float32)])
+    x = (mags * np.where(g.random(mags.size) < 0.5, -1, 1)).astype(np.float32)
+    for mode in oracle.STOCHASTIC_MODES:
+        for L in ((0,) if mode in (oracle.TRUESR, oracle.PROB) else (1, 5, 23 - mb)):
+            w = oracle.rand_u32(1000 + 31 * mode + L, x.size)
+            dec = oracle.rep_to_float(oracle.float_to_rep(x, pf, mode, stoch_len=L, bits=w), pf)
+            for xi, di, ui in zip(x.tolist(), dec.tolist(), w.tolist()):
+                q = Fraction(abs(xi)) / unit
+                fl = math.floor(q)
+                # an exact zero encodes as zero before any rounding (lo_pack.hpp:176)
+                want = 0 if xi == 0 else (fl + _stochastic_up(mode, q - fl, L, int(ui))) * unit
+                assert Fraction(abs(di)) == want and (di == 0 or (di < 0) == (xi < 0)), \
+                    (name, mode, L, xi, di, float(want))
+
+
+def test_stochastic_encode_unbiased_below_normal():
+    """TrueSR is unbiased: the mean of decode(encode(x)) over many words is x, also for
+    values below e4m3's smallest subnormal."""
+    pf = _pf((8, 3, 7, 1, 0, 0), "ocp_fn")
+    for v in (2.0 ** -10, 3 * 2.0 ** -12, 0.7 * 2.0 ** -9, 5.3 * 2.0 ** -9):
+        x = np.full(1 << 16, v, np.float32)
+        dec = oracle.rep_to_float(oracle.float_to_rep(x, pf, oracle.TRUESR, seed=5), pf)
+        sd = 2.0 ** -9 / 2 / math.sqrt(x.size)
+        assert abs(dec.astype(np.float64).mean() - float(np.float32(v))) < 5 * sd, v
