@@ -196,6 +196,44 @@ LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
                                      lof_stream_t stream, int32_t* best_config, float* best_ms);
 
 /* ---------------------------------------------------------------------------------
+ * Conv2d forward (cross-correlation, groups = 1) as an implicit GEMM on the same
+ * emulated-accumulation leaf: M = K (output channels), N = batch * P * Q (output
+ * positions), reduction over the filter taps, B gathered from x with zero padding, never
+ * materialised.  P = (H + 2 pad_h - dil_h (R - 1) - 1) / stride_h + 1, likewise Q.
+ *   LOF_CONV_NCHW: x N x C x H x W, w K x C x R x S (OIHW), y and c N x K x P x Q; the
+ *     reduction index is (c, r, s) with s fastest -- w viewed as a K x CRS matrix times
+ *     unfold(x), the order of LoF_Conv2d's im2col + lof_gemm path (layers.py:687-737).
+ *   LOF_CONV_NHWC: x N x H x W x C, w K x R x S x C (KRSC), y and c N x P x Q x K; the
+ *     reduction index is (r, s, c) with c fastest.
+ * Each output is the ascending-k leaf chain of lof_gemm_f32 over its reduction.  Only the
+ * wave-tile configs run convolutions (chunk in {1, 2, 4, 8, 16, 32}); x must have fewer
+ * than 2^31 elements.
+ * ------------------------------------------------------------------------------- */
+enum { LOF_CONV_NCHW = 0, LOF_CONV_NHWC = 1 };
+
+typedef struct lof_conv2d_desc {
+    int32_t N, H, W, C;          /* input batch, height, width, channels */
+    int32_t K, R, S;             /* output channels, filter height, width */
+    int32_t pad_h, pad_w, stride_h, stride_w, dil_h, dil_w;
+    int32_t layout;              /* LOF_CONV_NCHW or LOF_CONV_NHWC */
+    const float* x;
+    const float* w;
+    float* y;
+    float alpha;
+    lof_accum accum;
+    const float* c;              /* read when beta != 0; y's layout */
+    float beta;
+} lof_conv2d_desc;
+
+/* y = alpha * conv(x, w) [+ beta * c].  config: -1 for the default wave tile, else a
+ * wave-tile config index (lof_gemm_config_valid).  Replaces lo_float::LoF_conv2d
+ * (Lof_kernel.cu:156-228; NCHW/OIHW, alpha = 1 / (weight_scale * input_scale), beta = 0)
+ * and lo_float::Conv2d::operator() (cutlass_conv.cuh:118-181; NHWC/KRSC).  The reference's
+ * CUTLASS mainloop fixes its own k order; the order here is the one stated above. */
+LOF_API lof_status lof_conv2d_f32(const lof_conv2d_desc* desc, int32_t config,
+                                  lof_stream_t stream);
+
+/* ---------------------------------------------------------------------------------
  * Grouped persistent launch: many rounding jobs and GEMMs, each with its own format
  * and mode, in ONE kernel launch that walks a device work queue.
  * ------------------------------------------------------------------------------- */

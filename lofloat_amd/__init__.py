@@ -1,7 +1,7 @@
 """lofloat_amd -- MI355X (gfx950) implementation of LoFloat's data-parallel hot path.
 
 Drop-in names of the reference package ``LoFloat`` (LoFloat/__init__.py:1-5):
-``virtual_round``, ``virtual_mx_round``, ``lof_gemm``, ``RoundingMode``, ``InfBehavior``,
+``virtual_round``, ``virtual_mx_round``, ``lof_gemm``, ``lof_conv2d``, ``RoundingMode``, ``InfBehavior``,
 ``NaNBehavior``, ``Signedness``, ``FloatFormatDescriptor``, the ``create_*_params``
 factories, the ``torch.ops.lofloat.*`` operators and the layers; ``lofloat_amd.pack`` is
 the device bit-pack codec of lo_pack.hpp.  The arithmetic runs in
@@ -15,10 +15,9 @@ from .formats import (create_bfloat16_params, create_e8m0_params,  # noqa: F401
                       create_half_params, create_ocp_e2m1_params, create_ocp_e3m2_params,
                       create_ocp_e4m3fn_params, create_ocp_e5m2_params, create_p3109_params,
                       create_single_params, make_format)
-from .ops import (AccumSpec, fill_random_u32, gemm, gemm_autotune,  # noqa: F401
-                  gemm_config_name, gemm_config_valid, gemm_num_configs, lof_gemm, set_seed,
-                  virtual_mx_round,
-                  virtual_round)
+from .ops import (AccumSpec, conv2d, fill_random_u32, gemm, gemm_autotune,  # noqa: F401
+                  gemm_config_name, gemm_config_valid, gemm_num_configs, lof_conv2d,
+                  lof_gemm, set_seed, virtual_mx_round, virtual_round)
 from . import _custom_ops  # noqa: F401,E402  (registers torch.ops.lofloat.*)
 from ._custom_ops import get_format, register_format  # noqa: F401,E402
 from .layers import (ExplicitMultiheadAttention, FISRBatchNorm, L1BatchNorm,  # noqa: F401,E402
@@ -42,5 +41,4 @@ def _out_of_scope(name: str):
     return f
 
 
-lof_conv2d = _out_of_scope("lof_conv2d")
 pwl_silu = _out_of_scope("pwl_silu")

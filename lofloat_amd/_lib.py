@@ -44,6 +44,15 @@ class GemmDesc(ctypes.Structure):
                 ("beta", ctypes.c_float)]
 
 
+class Conv2dDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("N", "H", "W", "C", "K", "R", "S", "pad_h", "pad_w", "stride_h", "stride_w",
+                 "dil_h", "dil_w", "layout")] + [
+        ("x", ctypes.c_void_p), ("w", ctypes.c_void_p), ("y", ctypes.c_void_p),
+        ("alpha", ctypes.c_float), ("accum", Accum), ("c", ctypes.c_void_p),
+        ("beta", ctypes.c_float)]
+
+
 class GroupItem(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p),
                 ("n", ctypes.c_int64), ("fmt", Format), ("mant_bits", ctypes.c_int32),
@@ -77,6 +86,7 @@ SIGNATURES = {
     "lof_virtual_mx_round_f32": (_i32, [_vp, _vp, _i64, _i32, _P(Format), _P(Format), RoundSpec,
                                         _i32, _vp, _u64, _vp]),
     "lof_gemm_f32": (_i32, [_P(GemmDesc), _i32, _vp]),
+    "lof_conv2d_f32": (_i32, [_P(Conv2dDesc), _i32, _vp]),
     "lof_gemm_num_configs": (_i32, []),
     "lof_gemm_config_valid": (_i32, [_i32, _i32]),
     "lof_gemm_config_name": (_i32, [_i32, ctypes.c_char_p, _i32]),

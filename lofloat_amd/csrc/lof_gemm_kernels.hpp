@@ -28,6 +28,45 @@
 
 namespace lofamd {
 
+// Division of a 32-bit unsigned n by an invariant d >= 1 (Granlund & Montgomery, "Division
+// by invariant integers using multiplication", 1994, Fig. 4.1): l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(m, n), q = (t + ((n - t) >> s1)) >> s2 with
+// s1 = min(l, 1), s2 = max(l - 1, 0) (so d = 1 gives m = 1, q = n).
+struct FastDiv {
+    uint32_t m;
+    int32_t s1, s2;
+};
+inline FastDiv make_fast_div(uint32_t d) {
+    int l = 0;
+    while (l < 32 && ((uint64_t)1 << l) < d) ++l;
+    FastDiv f;
+    f.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1u);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 1 ? l - 1 : 0;
+    return f;
+}
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv& f) {
+    const uint32_t t = __umulhi(n, f.m);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+// Implicit-GEMM convolution (cross-correlation): B(k, n) is gathered from an activation
+// tensor and D(m, n) / C(m, n) live at strided output positions.  k = (d1, d2, d3) with
+// d1 = k / D1, d2 = (k % D1) / D2, d3 = k % D2: (c, r, s) for an OIHW filter, (r, s, c) for
+// KRSC.  n = (b, p, q) = output image b, row p, column q.  conv = 0 for a plain GEMM.
+struct ConvGeom {
+    int32_t conv;
+    int32_t krsc;                 // 0: k = (c, r, s) (OIHW), 1: k = (r, s, c) (KRSC)
+    int32_t H, W, P, Q;
+    int32_t ph, pw, sh, sw, dh, dw;
+    int32_t D2;                   // S (OIHW) or C (KRSC)
+    int32_t sxc, sxh, sxw;        // activation strides (elements); batch stride sxb
+    int64_t sxb;
+    int64_t sdb, sdm, sdpq;       // output strides: batch, output channel m, position p*Q+q
+    FastDiv div_pq, div_q, div_d1, div_d2;
+    uint32_t PQ, D1;
+};
+
 struct GemmArgs {
     int32_t M, N, K;
     const float* A;
@@ -65,6 +104,7 @@ struct GemmArgs {
     int32_t vec;         // A, B 16-byte aligned with lda, ldb multiples of 4 (float4 loads)
     int32_t mfma_mode;   // MFMA tile: 0 exactness test, 1 every chunk on the VALU chain,
                          // 2 every chunk on MFMA (testing only: skips the test)
+    ConvGeom cv;         // wave tiles only: B gathered from an activation (cv.conv != 0)
 };
 
 __device__ __forceinline__ uint32_t next_up_bits(uint32_t b) {

@@ -156,7 +156,10 @@ __device__ __forceinline__ bool slice_bounds(uint32_t amax, uint32_t mnk, uint32
     return emin_b != 0;
 }
 
-template <int WMB, int WNB, int CHUNK, bool SUBN>
+// CONV: B is gathered from g.cv's activation and D / C are stored at its strided output
+// positions (implicit-GEMM convolution); a separate instantiation, so the plain GEMM
+// kernels carry none of its registers.
+template <int WMB, int WNB, int CHUNK, bool SUBN, bool CONV = false>
 __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, int tile_n,
                                                int tid, float* smem) {
     using T = MfmaTile<WMB, WNB>;
@@ -197,8 +200,60 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         if (g.tb) { n = f / 8; k = 4 * (f % 8) + j; }
         else { k = f / (BN / 4); n = 4 * (f % (BN / 4)) + j; }
     };
+    // Convolution: this thread's four B columns n = n0 + 4 * (tid % (BN / 4)) + j are the
+    // same in every slice (256 is a multiple of BN / 4); their output image, first input row
+    // and column and base offset are computed once.  A column past N gets a row that puts
+    // every tap outside the image.
+    int cv_ih0[4], cv_iw0[4], cv_base[4];
+    if constexpr (CONV) {
+        const ConvGeom& cv = g.cv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gn = n0 + 4 * (tid % (BN / 4)) + j;
+            const uint32_t b = fast_div((uint32_t)gn, cv.div_pq);
+            const uint32_t pq = (uint32_t)gn - b * cv.PQ;
+            const uint32_t p = fast_div(pq, cv.div_q);
+            const uint32_t q = pq - p * (uint32_t)cv.Q;
+            const bool valid = gn < g.N;
+            cv_ih0[j] = valid ? (int)p * cv.sh - cv.ph : -(1 << 30);
+            cv_iw0[j] = (int)q * cv.sw - cv.pw;
+            cv_base[j] = valid ? (int)((int64_t)b * cv.sxb) + cv_ih0[j] * cv.sxh + cv_iw0[j] * cv.sxw
+                               : 0;
+        }
+    }
+    // B(k, n) for the slice rows k = k0 + f / (BN / 4): the tap (c, r, s) of k, then one
+    // element per column j, zero where the tap falls in the padding or k >= K
+    auto gather_b = [&](int k0) {
+        const ConvGeom& cv = g.cv;
+#pragma unroll
+        for (int it = 0; it < NB4; ++it) {
+            int n, k;
+            b_coord(it, 0, n, k);
+            const int gk = k0 + k;
+            const uint32_t d1 = fast_div((uint32_t)gk, cv.div_d1);
+            const uint32_t rem = (uint32_t)gk - d1 * cv.D1;
+            const uint32_t d2 = fast_div(rem, cv.div_d2);
+            const uint32_t d3 = rem - d2 * (uint32_t)cv.D2;
+            const int c = cv.krsc ? (int)d3 : (int)d1;
+            const int r = cv.krsc ? (int)d1 : (int)d2;
+            const int sx = cv.krsc ? (int)d2 : (int)d3;
+            const int rr = r * cv.dh, ss = sx * cv.dw;
+            const int koff = c * cv.sxc + rr * cv.sxh + ss * cv.sxw;
+            float e[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int ih = cv_ih0[j] + rr, iw = cv_iw0[j] + ss;
+                const bool in = gk < g.K && (unsigned)ih < (unsigned)cv.H &&
+                                (unsigned)iw < (unsigned)cv.W;
+                const float v = g.B[in ? cv_base[j] + koff : 0];   // offset 0 when outside
+                e[j] = in ? v : 0.0f;
+            }
+            vb[it] = make_float4(e[0], e[1], e[2], e[3]);
+        }
+    };
     auto load_slice = [&](int k0) {
-        if (g.vec && interior_mn && k0 + BK <= g.K) {   // whole slice inside: float4 loads
+        const bool whole = g.vec && interior_mn && k0 + BK <= g.K;
+        if (whole) {   // whole slice inside: float4 loads
 #pragma unroll
             for (int it = 0; it < NA4; ++it) {
                 int m, k;
@@ -206,14 +261,6 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 const float* p = g.ta ? g.A + (int64_t)(k0 + k) * g.lda + (m0 + m)
                                       : g.A + (int64_t)(m0 + m) * g.lda + (k0 + k);
                 va[it] = *(const float4*)p;
-            }
-#pragma unroll
-            for (int it = 0; it < NB4; ++it) {
-                int n, k;
-                b_coord(it, 0, n, k);
-                const float* p = g.tb ? g.B + (int64_t)(n0 + n) * g.ldb + (k0 + k)
-                                      : g.B + (int64_t)(k0 + k) * g.ldb + (n0 + n);
-                vb[it] = *(const float4*)p;
             }
         } else {   // edge: element loads from clamped addresses, zero outside the matrix
 #pragma unroll
@@ -230,6 +277,19 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 }
                 va[it] = make_float4(e[0], e[1], e[2], e[3]);
             }
+        }
+        if constexpr (CONV) {
+            gather_b(k0);
+        } else if (whole) {
+#pragma unroll
+            for (int it = 0; it < NB4; ++it) {
+                int n, k;
+                b_coord(it, 0, n, k);
+                const float* p = g.tb ? g.B + (int64_t)(n0 + n) * g.ldb + (k0 + k)
+                                      : g.B + (int64_t)(k0 + k) * g.ldb + (n0 + n);
+                vb[it] = *(const float4*)p;
+            }
+        } else {
 #pragma unroll
             for (int it = 0; it < NB4; ++it) {
                 float e[4];
@@ -464,21 +524,28 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         if (has_next) store_slice(buf ^ 1);
     }
 
-    // epilogue (cutlass LinearCombination): D = alpha * acc, or fma(alpha, acc, beta * C)
+    // epilogue (cutlass LinearCombination): D = alpha * acc, or fma(alpha, acc, beta * C);
+    // a convolution's D(m, n) / C(m, n) sit at b * sdb + m * sdm + (p * Q + q) * sdpq
 #pragma unroll
     for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
         for (int bj = 0; bj < WNB; ++bj) {
             const int gn = n0 + bn0 + bj * 32 + l32;
+            int64_t col = gn, row_stride_d = g.ldd, row_stride_c = g.ldc;
+            if constexpr (CONV) {
+                const uint32_t b = fast_div((uint32_t)gn, g.cv.div_pq);
+                col = (int64_t)b * g.cv.sdb + (int64_t)((uint32_t)gn - b * g.cv.PQ) * g.cv.sdpq;
+                row_stride_d = row_stride_c = g.cv.sdm;
+            }
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int gm = m0 + am0 + bi * 32 + (r >> 2) * 8 + lhi * 4 + (r & 3);
                 if (gm >= g.M || gn >= g.N) continue;
                 const float a = acc[bi][bj][r];
                 float d;
-                if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * g.C[(int64_t)gm * g.ldc + gn]);
+                if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * g.C[(int64_t)gm * row_stride_c + col]);
                 else d = g.alpha * a;
-                g.D[(int64_t)gm * g.ldd + gn] = d;
+                g.D[(int64_t)gm * row_stride_d + col] = d;
             }
         }
 }

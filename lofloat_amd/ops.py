@@ -242,6 +242,97 @@ def lof_gemm(A: torch.Tensor, B: torch.Tensor, accum_mant_bits: int,
     return gemm(A, B, AccumSpec(mant_bits=int(accum_mant_bits), chunk=1), alpha=alpha)
 
 
+def _pair(v, what: str):
+    if isinstance(v, int):
+        return int(v), int(v)
+    t = tuple(int(e) for e in v)
+    if len(t) != 2:
+        raise ValueError(f"{what}: expected an int or a pair")
+    return t
+
+
+def conv_out_size(n: int, pad: int, dil: int, k: int, stride: int) -> int:
+    return (n + 2 * pad - dil * (k - 1) - 1) // stride + 1
+
+
+def conv2d(x: torch.Tensor, w: torch.Tensor, accum: AccumSpec, stride=1, padding=0,
+           dilation=1, alpha: float = 1.0, layout: str = "NCHW",
+           C: torch.Tensor | None = None, beta: float = 0.0,
+           out: torch.Tensor | None = None, config: int = -1) -> torch.Tensor:
+    """Conv2d forward (cross-correlation, groups = 1) as an implicit GEMM with the emulated
+    accumulator: y = alpha * conv(x, w) (+ beta * C).  ``layout="NCHW"``: x N x C x H x W,
+    w K x C x R x S, y N x K x P x Q, reduction over (c, r, s) with s fastest -- the
+    order of the filter-as-matrix times unfold(x) GEMM; ``"NHWC"``: x N x H x W x C,
+    w K x R x S x C, y N x P x Q x K, reduction over (r, s, c).  Each output is the
+    ascending-k leaf chain of ``gemm`` over its reduction; the unfolded input is never
+    materialised."""
+    _require_device_f32(x, "conv2d")
+    _require_device_f32(w, "conv2d")
+    if x.dim() != 4 or w.dim() != 4:
+        raise RuntimeError("conv2d: input and weight must be 4-D")
+    if x.device != w.device:
+        raise RuntimeError("conv2d: input and weight must be on the same device")
+    nhwc = {"NCHW": False, "NHWC": True}.get(layout)
+    if nhwc is None:
+        raise ValueError(f"conv2d: layout must be 'NCHW' or 'NHWC' (got {layout!r})")
+    (sh, sw), (ph, pw), (dh, dw) = (_pair(stride, "stride"), _pair(padding, "padding"),
+                                    _pair(dilation, "dilation"))
+    if nhwc:
+        N, H, W, Cin = x.shape
+        K, R, S, Cw = w.shape
+    else:
+        N, Cin, H, W = x.shape
+        K, Cw, R, S = w.shape
+    if Cw != Cin:
+        raise RuntimeError(f"conv2d: input channels ({Cin}) != weight channels ({Cw})")
+    P, Q = conv_out_size(H, ph, dh, R, sh), conv_out_size(W, pw, dw, S, sw)
+    if P < 1 or Q < 1:
+        raise RuntimeError(f"conv2d: non-positive output spatial size (P={P}, Q={Q})")
+    shape = (N, P, Q, K) if nhwc else (N, K, P, Q)
+    y = torch.empty(shape, dtype=torch.float32, device=x.device) if out is None else out
+    if tuple(y.shape) != shape or not y.is_contiguous() or y.dtype != torch.float32:
+        raise ValueError(f"conv2d: out must be a contiguous float32 tensor of shape {shape}")
+    xc, wc = x.contiguous(), w.contiguous()
+    Cc = None
+    if beta != 0.0:
+        if C is None or tuple(C.shape) != shape:
+            raise ValueError(f"conv2d: beta != 0 needs C of shape {shape}")
+        _require_device_f32(C, "conv2d C")
+        Cc = C.contiguous()
+    desc = _lib.Conv2dDesc(N, H, W, Cin, K, R, S, ph, pw, sh, sw, dh, dw, int(nhwc),
+                           xc.data_ptr(), wc.data_ptr(), y.data_ptr(), float(alpha),
+                           accum.c_accum(), Cc.data_ptr() if Cc is not None else None,
+                           float(beta))
+    with torch.cuda.device(x.device):
+        st = _lib.lib.lof_conv2d_f32(ctypes.byref(desc), int(config), _stream(x))
+    _lib.check(st, "conv2d")
+    return y
+
+
+def lof_conv2d(input: torch.Tensor, weight: torch.Tensor, pad_h: int, pad_w: int,
+               stride_h: int, stride_w: int, dilation_h: int, dilation_w: int,
+               accum_mant_bits: int, round_mode=RoundingMode.RoundToNearestEven,
+               stochastic_rounding_bits: int = 0, weight_scale: float = 1.0,
+               input_scale: float = 1.0) -> torch.Tensor:
+    """The reference's lof_conv2d (Lof_kernel.cu:156-228; LoPy_bind.cpp:374-392): NCHW
+    input, OIHW weight, NCHW output divided by (weight_scale * input_scale) (no division
+    when that product is 0), accumulator re-rounded to ``accum_mant_bits`` after every
+    multiply-add.
+    The conv runs the GEMM's leaf, which always rounds to nearest even, so ``round_mode``
+    and ``stochastic_rounding_bits`` are accepted and ignored as in lof_gemm."""
+    RoundingMode(int(round_mode))
+    if input.dim() != 4 or weight.dim() != 4:
+        raise RuntimeError("LoF_conv2d: input and weight must be 4-D (NCHW / OIHW)")
+    if input.size(1) != weight.size(1):
+        raise RuntimeError(f"LoF_conv2d: input C_in ({input.size(1)}) != weight C_in "
+                           f"({weight.size(1)})")
+    denom = float(weight_scale) * float(input_scale)          # in double, as the binding
+    alpha = 1.0 if denom == 0.0 else float(np.float32(1.0 / denom))
+    return conv2d(input, weight, AccumSpec(mant_bits=int(accum_mant_bits), chunk=1),
+                  stride=(stride_h, stride_w), padding=(pad_h, pad_w),
+                  dilation=(dilation_h, dilation_w), alpha=alpha)
+
+
 def gemm_num_configs() -> int:
     return int(_lib.lib.lof_gemm_num_configs())
 

@@ -261,6 +261,63 @@ def gemm(A, B, accum: _Accum, trans_a: bool = False, trans_b: bool = False,
     return out
 
 
+# ------------------------------------------------------------------ convolution
+def conv_out_size(n: int, pad: int, dil: int, k: int, stride: int) -> int:
+    return (n + 2 * pad - dil * (k - 1) - 1) // stride + 1
+
+
+def im2col(x, R: int, S: int, pad=(0, 0), stride=(1, 1), dil=(1, 1)) -> np.ndarray:
+    """unfold of an N x C x H x W array (torch.nn.functional.unfold's order): a C*R*S x N*P*Q
+    matrix, row (c, r, s) with s fastest, column (n, p, q) with q fastest, zero in the
+    padding."""
+    x = _f32(x)
+    N, C, H, W = x.shape
+    P = conv_out_size(H, pad[0], dil[0], R, stride[0])
+    Q = conv_out_size(W, pad[1], dil[1], S, stride[1])
+    xp = np.pad(x, ((0, 0), (0, 0), (pad[0], pad[0]), (pad[1], pad[1])))
+    cols = np.empty((C, R, S, N, P, Q), np.float32)
+    for r in range(R):
+        for s in range(S):
+            h0, w0 = r * dil[0], s * dil[1]
+            win = xp[:, :, h0:h0 + stride[0] * (P - 1) + 1:stride[0],
+                     w0:w0 + stride[1] * (Q - 1) + 1:stride[1]]
+            cols[:, r, s] = win.transpose(1, 0, 2, 3)
+    return cols.reshape(C * R * S, N * P * Q)
+
+
+def conv2d(x, w, accum: _Accum, pad=(0, 0), stride=(1, 1), dil=(1, 1), alpha: float = 1.0,
+           layout: int = 0, C=None, beta: float = 0.0, nthreads: int = 0) -> np.ndarray:
+    """Conv2d forward as LoF_Conv2d's im2col + lof_gemm path (layers.py:687-737): the
+    filter as a K x (reduction) matrix times the unfolded input, through the GEMM oracle.
+    layout 0: x NCHW, w KCRS (OIHW), result N x K x P x Q, reduction (c, r, s);
+    layout 1: x NHWC, w KRSC, result N x P x Q x K, reduction (r, s, c)."""
+    x = _f32(x)
+    w = _f32(w)
+    if layout == 0:
+        N = x.shape[0]
+        K, Cin, R, S = w.shape
+        cols = im2col(x, R, S, pad, stride, dil)
+        A = w.reshape(K, Cin * R * S)
+    else:
+        N = x.shape[0]
+        K, R, S, Cin = w.shape
+        cols = im2col(x.transpose(0, 3, 1, 2), R, S, pad, stride, dil)
+        cols = np.ascontiguousarray(cols.reshape(Cin, R, S, -1).transpose(1, 2, 0, 3)
+                                    .reshape(R * S * Cin, -1))
+        A = w.reshape(K, R * S * Cin)
+    P = conv_out_size(x.shape[2] if layout == 0 else x.shape[1], pad[0], dil[0], R, stride[0])
+    Q = conv_out_size(x.shape[3] if layout == 0 else x.shape[2], pad[1], dil[1], S, stride[1])
+    Cm = None
+    if beta != 0.0:
+        Cc = _f32(C)
+        Cm = (Cc.reshape(N, K, P * Q).transpose(1, 0, 2).reshape(K, N * P * Q) if layout == 0
+              else Cc.reshape(N * P * Q, K).T)
+    D = gemm(A, cols, accum, alpha=alpha, nthreads=nthreads, C=Cm, beta=beta)   # K x N*P*Q
+    if layout == 0:
+        return np.ascontiguousarray(D.reshape(K, N, P, Q).transpose(1, 0, 2, 3))
+    return np.ascontiguousarray(D.T.reshape(N, P, Q, K))
+
+
 # ------------------------------------------------------------------ bit-pack codec
 # Checker presets: the Inf / NaN functors of the reference's format parameter sets, as data
 # (kind, codes, a, b, pat0, pat1); see orc_checker in lof_oracle.h.
