@@ -148,3 +148,73 @@ def test_empty_and_errors(dev):
         lof.conv2d(x, torch.randn(4, 3, 9, 9, device=dev), lof.AccumSpec(mant_bits=10))
     with pytest.raises(NotImplementedError):            # stochastic accumulator
         lof.conv2d(x, w, lof.AccumSpec(mant_bits=10, mode=3))
+
+
+# ----------------------------------------------------------------------- LoF_Conv2d layer
+def _layer_ref(layer, x, fmt_t):
+    """The reference layer's forward composed from oracle pieces (layers.py:620-750)."""
+    w = layer.weight.detach().cpu().numpy()
+    pad, stride, dil = layer.padding, layer.stride, layer.dilation
+    acc = oracle.accum_mant(layer.accum_mant_bits, 1)
+    if layer.scaling == "mx":
+        B = x.shape[0]
+        cols = oracle.im2col(x, *layer.kernel_size, pad, stride, dil)          # K x B*L
+        K = cols.shape[0]
+        rows = np.ascontiguousarray(cols.T)                                   # (b, l) x K
+        sf = fmt_t(layer.scale_format)
+        rows = oracle.virtual_mx_round(rows.ravel(), layer.mx_block_size, fmt_t(layer.act_params),
+                                       sf).reshape(rows.shape)
+        wq = oracle.virtual_mx_round(w.reshape(w.shape[0], -1).ravel(), layer.mx_block_size,
+                                     fmt_t(layer.weight_params), sf).reshape(w.shape[0], K)
+        D = oracle.gemm(wq, np.ascontiguousarray(rows.T), acc)
+        P = oracle.conv_out_size(x.shape[2], pad[0], dil[0], layer.kernel_size[0], stride[0])
+        Q = oracle.conv_out_size(x.shape[3], pad[1], dil[1], layer.kernel_size[1], stride[1])
+        out = D.reshape(w.shape[0], B, P, Q).transpose(1, 0, 2, 3)
+        bq = oracle.virtual_round(layer.bias.detach().cpu().numpy(), fmt_t(layer.bias_params))
+        return out + bq.reshape(1, -1, 1, 1)
+    xq = oracle.virtual_round(x.ravel(), fmt_t(layer.act_params),
+                              scale=layer.act_scale_factor).reshape(x.shape)
+    wq = oracle.virtual_round(w.ravel(), fmt_t(layer.weight_params),
+                              scale=layer.w_scale_factor).reshape(w.shape)
+    if layer.backend == "cutlass":
+        alpha = float(np.float32(1.0 / (layer.w_scale_factor * layer.act_scale_factor)))
+    else:
+        alpha = float(np.float32(1.0) / (np.float32(layer.w_scale_factor) *
+                                          np.float32(layer.act_scale_factor)))
+    G = layer.groups
+    ci, co = x.shape[1] // G, w.shape[0] // G
+    out = np.concatenate([oracle.conv2d(np.ascontiguousarray(xq[:, g * ci:(g + 1) * ci]),
+                                        wq[g * co:(g + 1) * co], acc, pad, stride, dil,
+                                        alpha=alpha) for g in range(G)], axis=1)
+    bq = oracle.virtual_round(layer.bias.detach().cpu().numpy(), fmt_t(layer.bias_params),
+                              scale=layer.b_scale_factor) / np.float32(layer.b_scale_factor)
+    return out + bq.astype(np.float32).reshape(1, -1, 1, 1)
+
+
+@pytest.mark.parametrize("variant", ["im2col", "cutlass", "groups", "mx", "hadamard"])
+def test_lof_conv2d_layer(dev, variant):
+    import lofloat_amd as lof
+    e4m3 = lof.make_format(8, 3, 7)
+    kw = dict(act_params=e4m3, weight_params=e4m3, bias_params=lof.make_format(16, 10, 15),
+              accum_mant_bits=10, padding=1)
+    if variant == "cutlass":
+        kw.update(backend="cutlass", act_scale_factor=3.0, w_scale_factor=7.0)
+    elif variant == "im2col":
+        kw.update(act_scale_factor=3.0, w_scale_factor=7.0, b_scale_factor=2.0, stride=2)
+    elif variant == "groups":
+        kw.update(groups=2, dilation=2)
+    elif variant == "mx":
+        kw.update(scaling="mx", mx_block_size=16)
+    else:
+        kw.update(hadamard_transform=True)
+    torch.manual_seed(0)
+    layer = lof.LoF_Conv2d(16, 12, 3, **kw).to(dev)
+    with torch.no_grad():
+        layer.bias.copy_(torch.randn(12))
+    x = torch.randn(2, 16, 9, 10)
+    y = layer(x.to(dev)).detach().cpu().numpy()
+    xin = x.numpy()
+    if variant == "hadamard":
+        xin = lof._fwht(x.movedim(1, -1).contiguous()).movedim(-1, 1).contiguous().numpy()
+    ref = _layer_ref(layer, xin, lambda p: p.as_tuple())
+    assert_bits(y, ref.astype(np.float32), f"LoF_Conv2d {variant}")
