@@ -11,7 +11,7 @@ with e2m1 and with SR-A random words; the emulated-accumulation GEMM at M=N=K=40
 LoF_Linear forward+backward step (X 8192x4096, W 4096x4096); and the grouped launch of 64
 uneven rounding jobs + 12 GEMMs (e4m3 operands) against per-job launches, both captured in
 HIP graphs; the bit-pack codec encode / decode of 2^28 values to e4m3fn and to an 11-bit
-P3109 format.
+P3109 format; a 3x3 Conv2d layer as an implicit GEMM against unfold + GEMM.
 
 The CPU baseline is the C oracle (a restatement of the reference's scalar semantics) run
 with OpenMP on the host cores on the same workload.
@@ -75,11 +75,12 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run N device copies of the 8192x8192 matrix (a plain copy "
                          "kernel of known traffic, to calibrate HBM byte counters)")
-    ap.add_argument("--only", choices=["gemm", "linear", "grouped", "pack"], default=None,
+    ap.add_argument("--only", choices=["gemm", "linear", "grouped", "pack", "conv"], default=None,
                     help="run one workload alone (for profiling): 'gemm' = the 4096^3 "
                          "emulated-accumulation GEMM with --chunk / --accum / --inputs; "
                          "'linear' = the LoF_Linear step; 'grouped' = the batched launch; "
-                         "'pack' = the bit-pack codec encode / decode of 2^28 elements")
+                         "'pack' = the bit-pack codec encode / decode of 2^28 elements; "
+                         "'conv' = the implicit-GEMM Conv2d")
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--accum", choices=["mant10", "5e10m"], default="5e10m")
     ap.add_argument("--inputs", choices=["e4m3", "8e7m"], default="e4m3")
@@ -103,11 +104,11 @@ def main():
     if args.only == "gemm":
         gemm_only(lof, torch, dev, args)
         return
-    if args.only in ("linear", "grouped", "pack"):
+    if args.only in ("linear", "grouped", "pack", "conv"):
         import numpy as np
         out = {}
-        {"linear": linear_steps, "grouped": grouped_vs_separate,
-         "pack": pack_codec}[args.only](lof, torch, np, dev, out)
+        {"linear": linear_steps, "grouped": grouped_vs_separate, "pack": pack_codec,
+         "conv": conv_layer}[args.only](lof, torch, np, dev, out)
         print(json.dumps(out), flush=True)
         return
 
@@ -257,7 +258,42 @@ def extras(lof, torch, np, oracle, dev):
     linear_steps(lof, torch, np, dev, out)
     grouped_vs_separate(lof, torch, np, dev, out)
     pack_codec(lof, torch, np, dev, out)
+    conv_layer(lof, torch, np, dev, out)
     return out
+
+
+def conv_layer(lof, torch, np, dev, out):
+    """Conv2d forward, 3x3 / pad 1, 128 -> 128 channels on 32 x 28 x 28 (a ResNet-50 stage-2
+    layer): M = 128, N = 25088, K = 1152, 7.4 GFLOP.  Inputs pre-rounded to e4m3.  Timed:
+    the implicit GEMM (input gathered into LDS, no unfold in HBM) for a 5e10m accumulator
+    at chunk 32 and for the reference lof_conv2d's 10-bit accumulator at chunk 1; and the
+    reference layer's route for the same math, unfold in HBM then the GEMM."""
+    e4m3 = lof.make_format(8, 3, 7)
+    g = torch.Generator(device=dev).manual_seed(11)
+    x = lof.virtual_round(torch.randn(32, 128, 28, 28, device=dev, generator=g) / 8, e4m3)
+    w = lof.virtual_round(torch.randn(128, 128, 3, 3, device=dev, generator=g) / 8, e4m3)
+    flop = 2.0 * 128 * (32 * 28 * 28) * 1152
+    w_col = w.reshape(128, -1)
+    for name, spec, it in (("5e10m_c32", lof.AccumSpec(fmt=lof.make_format(16, 10, 15), chunk=32), 10),
+                           ("mant10_c1", lof.AccumSpec(mant_bits=10, chunk=1), 3)):
+        ms = timed(lambda i: lof.conv2d(x, w, spec, padding=1), it)
+        out[f"conv3x3_128_28x28_b32_{name}_TFLOPs"] = round(flop / ms / 1e9, 3)
+
+        def unfold_gemm(i):
+            cols = torch.nn.functional.unfold(x, 3, padding=1)            # 32 x 1152 x 784
+            cols = cols.permute(1, 0, 2).reshape(1152, -1)                 # K x N, copied
+            return lof.gemm(w_col, cols, spec)
+        ms_u = timed(unfold_gemm, it)
+        out[f"conv3x3_128_28x28_b32_{name}_unfold_gemm_TFLOPs"] = round(flop / ms_u / 1e9, 3)
+        log(f"conv {name}: implicit {ms:.2f} ms ({flop / ms / 1e9:.1f} TFLOP/s), "
+            f"unfold + gemm {ms_u:.2f} ms ({flop / ms_u / 1e9:.1f} TFLOP/s)")
+    y = lof.conv2d(x, w, lof.AccumSpec(fmt=lof.make_format(16, 10, 15), chunk=32), padding=1)
+    cols = torch.nn.functional.unfold(x, 3, padding=1).permute(1, 0, 2).reshape(1152, -1)
+    y2 = lof.gemm(w_col, cols, lof.AccumSpec(fmt=lof.make_format(16, 10, 15), chunk=32))
+    out["conv_implicit_equals_unfold_gemm"] = bool(torch.equal(
+        y, y2.view(128, 32, 28, 28).permute(1, 0, 2, 3)))
+    del x, w, y, y2, cols
+    torch.cuda.empty_cache()
 
 
 def pack_codec(lof, torch, np, dev, out):

@@ -117,9 +117,9 @@ LOF_API lof_status lof_conv2d_f32(const lof_conv2d_desc* desc, int32_t config,
                          "{1, 2, 4, 8, 16, 32} (got %d)", what, g.chunk);
     int tile;   // 0: 64 x 64, 1: 128 x 64
     const int cfg = config >= 0 ? (config & 0xFF) : -1;
-    if (cfg < 0) {   // the GEMM's default rule: 128 x 64 for long chunks with >= 1024 tiles
+    if (cfg < 0) {   // the GEMM's default rule: 128 x 64 for long chunks with >= 384 tiles
         const int64_t tiles_128x64 = ((int64_t)c.K + 127) / 128 * ((Ncols + 63) / 64);
-        tile = (g.chunk >= 16 && tiles_128x64 >= 1024) ? 1 : 0;
+        tile = (g.chunk >= 16 && tiles_128x64 >= 384) ? 1 : 0;
     } else if (cfg == kCfgMfma11) {
         tile = 0;
     } else if (cfg == kCfgMfma21) {

@@ -95,14 +95,15 @@ TuneKey tune_key(const lof_gemm_desc& d) {
                    d.accum.chunk};
 }
 
-// Untuned choice, from measurements at M = N = K = 4096 (e4m3 / 8e7m inputs): wave tiles
-// for the chunks they take (64x64 for c <= 8, where the leaves dominate, chunk 1 included;
-// 128x64 for c >= 16 when that still gives >= 1024 tiles), the 32x64 VALU tile otherwise;
-// the 32x32 VALU tile when there are too few tiles to fill the chip.
+// Untuned choice, from measurements at M = N = K = 4096 and on conv-shaped GEMMs
+// (M = 64..512, e4m3 / 8e7m inputs): wave tiles for the chunks they take (64x64 for c <= 8,
+// where the leaves dominate, chunk 1 included; 128x64 for c >= 16 when that still gives
+// >= 384 tiles, 1.5 per CU), the 32x64 VALU tile otherwise; the 32x32 VALU tile when there
+// are too few tiles to fill the chip.
 int default_config(const lof_gemm_desc& d) {
     if (chunk_fits_wave_tile(d.accum.chunk)) {
         const int64_t tiles_128x64 = (int64_t)((d.M + 127) / 128) * ((d.N + 63) / 64);
-        return (d.accum.chunk >= 16 && tiles_128x64 >= 1024) ? kMfma128x64 : kMfma64;
+        return (d.accum.chunk >= 16 && tiles_128x64 >= 384) ? kMfma128x64 : kMfma64;
     }
     if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 1024) return 3;
     return 4;

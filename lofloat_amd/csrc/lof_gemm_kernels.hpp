@@ -179,6 +179,16 @@ __device__ __forceinline__ bool leaf_fast(double p, double acc, const GemmArgs& 
     return ((hi & 0x7FFFFFFFu) - g.lo_bound) < g.range;
 }
 
+// The leaf of an exactly-zero sum while the fast path is on (round to nearest even, which
+// keeps a zero): RTO24 gives +0 for (+0) + (+0) and -0 for any other exact zero
+// (roundTowardNegative).  Outside the fast domain, so checked in the slow branch, where it
+// spares the general path the zero sums of padded tile rows / columns and zero data.
+__device__ __forceinline__ bool leaf_zero(double p, double acc, const GemmArgs& g, double& out) {
+    if (g.range == 0u || p + acc != 0.0) return false;
+    out = (__double_as_longlong(p) | __double_as_longlong(acc)) == 0 ? 0.0 : -0.0;
+    return true;
+}
+
 __device__ __noinline__ double leaf_slow(double p, double acc, const GemmArgs& g) {
     const double s = p + acc;
     const double bb = s - p;
@@ -192,7 +202,7 @@ __device__ __noinline__ double leaf_slow(double p, double acc, const GemmArgs& g
 __device__ __forceinline__ double leaf(double p, double acc, const GemmArgs& g) {
     double r;
     const bool ok = g.subn ? leaf_fast<true>(p, acc, g, r) : leaf_fast<false>(p, acc, g, r);
-    if (__builtin_expect(!ok, 0)) r = leaf_slow(p, acc, g);
+    if (__builtin_expect(!ok, 0) && !leaf_zero(p, acc, g, r)) r = leaf_slow(p, acc, g);
     return r;
 }
 
@@ -215,8 +225,11 @@ __device__ __forceinline__ void leaves_d(const double* p, double* acc, const Gem
         }
         if (__builtin_expect(__any(slow != 0), 0)) {
 #pragma unroll
-            for (int o = g0; o < g0 + G && o < N; ++o)
-                if ((slow >> (o - g0)) & 1u) acc[o] = leaf_slow(p[o], acc[o], g);
+            for (int o = g0; o < g0 + G && o < N; ++o) {
+                double z;
+                if ((slow >> (o - g0)) & 1u)
+                    acc[o] = leaf_zero(p[o], acc[o], g, z) ? z : leaf_slow(p[o], acc[o], g);
+            }
         }
     }
 }

@@ -93,6 +93,13 @@ __device__ __forceinline__ bool leaf_f_fast(float p, float acc, const GemmArgs& 
     return ((sb & 0x7FFFFFFFu) - g.lo_f) < g.range_f;
 }
 
+// leaf_zero for a float sum (float addition cannot underflow to zero, so s == 0 is exact)
+__device__ __forceinline__ bool leaf_f_zero(float p, float acc, const GemmArgs& g, float& out) {
+    if (g.range_f == 0u || p + acc != 0.0f) return false;
+    out = (__float_as_uint(p) | __float_as_uint(acc)) == 0u ? 0.0f : -0.0f;
+    return true;
+}
+
 // Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
 // results are committed where valid, outputs needing the general path are recorded in a bit
 // mask, and one wave-uniform branch runs the general path for them (acc still holds their
@@ -109,8 +116,11 @@ __device__ __forceinline__ void leaves_f16(const f32x16& p, float* acc, const Ge
     }
     if (__builtin_expect(__any(slow != 0), 0)) {
 #pragma unroll
-        for (int o = 0; o < 16; ++o)
-            if ((slow >> o) & 1u) acc[o] = leaf_f_slow(p[o], acc[o], g);
+        for (int o = 0; o < 16; ++o) {
+            float z;
+            if ((slow >> o) & 1u)
+                acc[o] = leaf_f_zero(p[o], acc[o], g, z) ? z : leaf_f_slow(p[o], acc[o], g);
+        }
     }
 }
 

@@ -231,6 +231,36 @@ def test_zero_sign_rule(dev):
     assert np.signbit(y[0, 0]) and not np.signbit(y[1, 0]) and np.signbit(y[2, 0])
 
 
+@pytest.mark.parametrize("chunk", [1, 8, 32])
+def test_zero_sums_every_config(dev, chunk):
+    """Exact-zero sums on every tile config and both fast leaves: sparse small-integer
+    operands with +0 / -0 entries and column pairs that cancel exactly, so outputs hit
+    (+0)+(+0), -0 products and cancellations mid-chain; signs compared bit for bit."""
+    import lofloat_amd as lof
+    g = np.random.default_rng(chunk)
+    M, N, K = 70, 66, 96
+    A = g.integers(-3, 4, (M, K)).astype(np.float32)
+    A[g.random((M, K)) < 0.5] = 0.0
+    A[g.random((M, K)) < 0.1] = -0.0
+    A[:, 1:48:2] = -A[:, 0:48:2]                   # the first 24 k pairs cancel exactly
+    A[5] = 0.0
+    A[6] = -0.0
+    B = g.integers(-3, 4, (K, N)).astype(np.float32)
+    B[g.random((K, N)) < 0.3] = -0.0
+    B[1:48:2] = B[0:48:2]
+    B[:, 7] = 0.0
+    for acc, oacc in (_accums(lof, 10, chunk), _accums(lof, fmt=FP16_EXT, chunk=chunk)):
+        ref = _ref(A, B, oacc)
+        zeros = ref[ref == 0]
+        assert 0 < zeros.size < ref.size and np.signbit(zeros).any() and not np.signbit(zeros).all()
+        for cfg in range(lof.gemm_num_configs()):
+            if lof.gemm_config_valid(cfg, chunk):
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref,
+                            f"zero sums chunk {chunk} config {cfg}")
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x100), ref,
+                            f"zero sums chunk {chunk} config {cfg} general leaf")
+
+
 def test_alpha_and_reference_entry(dev):
     import lofloat_amd as lof
     A, B = _mats(64, 48, 96, seed=2)
