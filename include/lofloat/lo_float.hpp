@@ -7,6 +7,7 @@
 //   run_virtual_mx_round                          src/Lof_kernels.h:44-48, Lof_kernel.cu:70-94
 //   Layout, Matrix<T, idx, Layout>                src/layouts.h:16-19, src/Matrix.h:21-78
 //   Gemm<MatA, MatB, MatC, MatD>                  src/cutlass_gemms.cuh:1629-1896
+//   Conv2d<...>, Conv2dProblemSize                src/cutlass_conv.cuh:20-184
 //   PackEndian, LoF_packed_size, LoF_encode_cuda, LoF_decode_cuda and the checker
 //   functors they read                           src/lo_pack.hpp:50, 392-531; lo_float.h:1411-1789
 // Same names, argument meanings and enum values.  Differences: every launcher takes an
@@ -298,6 +299,109 @@ private:
         }
         return d;
     }
+};
+
+// ---------------------------------------------------------------------------------------
+// Conv2d forward (cutlass_conv.cuh:20-184).
+//
+// The problem description keeps the fields and constructor of CUTLASS's
+// conv::Conv2dProblemSize that lo_float::Conv2d reads; replace
+// `cutlass::conv::Conv2dProblemSize` / `cutlass::conv::Mode::kCrossCorrelation` by
+// `lo_float::Conv2dProblemSize` / `lo_float::ConvMode::kCrossCorrelation`.
+enum class ConvMode { kCrossCorrelation, kConvolution };
+
+struct Conv2dProblemSize {
+    int N, H, W, C, K, R, S, P, Q;
+    int pad_h, pad_w, stride_h, stride_w, dilation_h, dilation_w;
+    ConvMode mode;
+    int split_k_slices;
+    int groups;
+    Conv2dProblemSize(int N_, int H_, int W_, int C_, int K_, int R_, int S_, int P_, int Q_,
+                      int pad_h_, int pad_w_, int stride_h_, int stride_w_, int dilation_h_,
+                      int dilation_w_, ConvMode mode_ = ConvMode::kCrossCorrelation,
+                      int split_k_slices_ = 1, int groups_ = 1)
+        : N(N_), H(H_), W(W_), C(C_), K(K_), R(R_), S(S_), P(P_), Q(Q_), pad_h(pad_h_),
+          pad_w(pad_w_), stride_h(stride_h_), stride_w(stride_w_), dilation_h(dilation_h_),
+          dilation_w(dilation_w_), mode(mode_), split_k_slices(split_k_slices_),
+          groups(groups_) {}
+};
+
+// D = alpha * conv(A, B) + beta * C with A the NHWC activation (N, H, W, C), B the KRSC
+// filter (K, R, S, C), C and D NHWC outputs (N, P, Q, K), cross-correlation, groups = 1,
+// the accumulator re-rounded after every multiply-add.  Each output's reduction runs over
+// (r, s, c) with c fastest, in ascending order.  The CUTLASS tile parameters are accepted
+// for source compatibility; the tile is chosen by the library.  Extensions as in Gemm:
+// set_accum_format(), set_chunk().
+template <int TbM = 128, int TbN = 128, int TbK = 8, int WpM = 32, int WpN = 64, int WpK = 8,
+          int Stages = 3>
+class Conv2d {
+public:
+    using ElementA = float;
+    using ElementB = float;
+    using ElementC = float;
+    using ElementCompute = float;
+
+    enum class Status {
+        kSuccess = 0,
+        kErrorNotSupported = -1,
+        kErrorAllocation = -2,
+        kErrorInitialize = -3,
+        kErrorExecution = -4
+    };
+
+    Conv2d(int accum_mantissa_bits = 23, ProjSpec proj_spec = ProjSpec{})
+        : accum_mantissa_bits_(accum_mantissa_bits), proj_spec_(proj_spec) {}
+
+    void set_accum_mantissa_bits(int bits) { accum_mantissa_bits_ = bits; has_fmt_ = false; }
+    void set_proj_spec(ProjSpec spec) { proj_spec_ = spec; }
+    template <class I, class N>
+    void set_accum_format(FloatingPointParams<I, N> f) { fmt_ = detail::to_c(f); has_fmt_ = true; }
+    void set_chunk(int chunk) { chunk_ = chunk; }
+
+    Status operator()(const Conv2dProblemSize& ps, ElementA* dA, ElementB* dB, ElementC* dC,
+                      ElementC* dD, ElementCompute alpha = ElementCompute(1),
+                      ElementCompute beta = ElementCompute(0),
+                      hipStream_t stream = nullptr) const {
+        if (ps.groups != 1 || ps.split_k_slices != 1 || ps.mode != ConvMode::kCrossCorrelation)
+            return Status::kErrorNotSupported;
+        const auto out = [](int n, int pad, int dil, int k, int st) {
+            return st > 0 ? (n + 2 * pad - dil * (k - 1) - 1) / st + 1 : -1;
+        };
+        if (ps.P != out(ps.H, ps.pad_h, ps.dilation_h, ps.R, ps.stride_h) ||
+            ps.Q != out(ps.W, ps.pad_w, ps.dilation_w, ps.S, ps.stride_w))
+            return Status::kErrorNotSupported;
+        lof_conv2d_desc d{};
+        d.N = ps.N; d.H = ps.H; d.W = ps.W; d.C = ps.C;
+        d.K = ps.K; d.R = ps.R; d.S = ps.S;
+        d.pad_h = ps.pad_h; d.pad_w = ps.pad_w;
+        d.stride_h = ps.stride_h; d.stride_w = ps.stride_w;
+        d.dil_h = ps.dilation_h; d.dil_w = ps.dilation_w;
+        d.layout = LOF_CONV_NHWC;
+        d.x = dA;
+        d.w = dB;
+        d.y = dD;
+        d.alpha = alpha;
+        d.accum.kind = has_fmt_ ? LOF_ACCUM_FORMAT : LOF_ACCUM_MANTISSA;
+        d.accum.mant_bits = accum_mantissa_bits_;
+        d.accum.fmt = fmt_;
+        // the leaf rounds to nearest even whatever proj_spec says, as the GEMM's
+        // (cutlass_gemms.cuh:122); a full-format accumulator honours the mode
+        d.accum.spec = lof_round_spec{has_fmt_ ? (int32_t)proj_spec_.rounding_mode : 0, 0, 0};
+        d.accum.chunk = chunk_;
+        d.c = dC;
+        d.beta = beta;
+        const lof_status s = lof_conv2d_f32(&d, -1, (lof_stream_t)stream);
+        if (s == LOF_ERR_INVALID_ARGUMENT || s == LOF_ERR_UNSUPPORTED)
+            return Status::kErrorNotSupported;
+        return s == LOF_OK ? Status::kSuccess : Status::kErrorExecution;
+    }
+
+private:
+    int accum_mantissa_bits_;
+    ProjSpec proj_spec_;
+    lof_format fmt_{};
+    bool has_fmt_ = false;
+    int chunk_ = 1;
 };
 
 // ---------------------------------------------------------------------------------------

@@ -131,6 +131,16 @@ static void cpu_checks() {
     EXPECT(g.can_implement(A, B, C, D) == GemmR::Status::kErrorNotSupported,
            "shape mismatch rejected");
     EXPECT(g.get_workspace_size(A, B, C, D) == 0, "no workspace");
+    Conv2d<> conv(10);
+    using CS = Conv2d<>::Status;
+    EXPECT(conv(Conv2dProblemSize(1, 8, 8, 4, 6, 3, 3, 8, 8, 1, 1, 1, 1, 1, 1,
+                                  ConvMode::kCrossCorrelation, 1, 2),
+                nullptr, nullptr, nullptr, nullptr) == CS::kErrorNotSupported, "conv groups 2");
+    EXPECT(conv(Conv2dProblemSize(1, 8, 8, 4, 6, 3, 3, 7, 8, 1, 1, 1, 1, 1, 1),
+                nullptr, nullptr, nullptr, nullptr) == CS::kErrorNotSupported, "conv wrong P");
+    EXPECT(conv(Conv2dProblemSize(1, 8, 8, 4, 6, 3, 3, 8, 8, 1, 1, 1, 1, 1, 1,
+                                  ConvMode::kConvolution),
+                nullptr, nullptr, nullptr, nullptr) == CS::kErrorNotSupported, "conv mode");
     pack_cpu_checks();
 }
 
@@ -264,6 +274,55 @@ static void pack_gpu_checks(std::mt19937& rng) {
         P_3109_NaNChecker<6, Unsigned>{}, NegtoNaN), x);
 }
 
+// Conv2d (NHWC activation, KRSC filter, NHWC output) against the oracle GEMM of the filter
+// (K x RSC) with the unfolded input, reduction order (r, s, c); D = alpha * conv + beta * C.
+static void conv_case(std::mt19937& rng) {
+    const int N = 2, H = 11, W = 9, C = 5, K = 12, R = 3, S = 2, ph = 1, pw = 0, sh = 2, sw = 1,
+              dh = 1, dw = 2;
+    const int P = (H + 2 * ph - dh * (R - 1) - 1) / sh + 1, Q = (W + 2 * pw - dw * (S - 1) - 1) / sw + 1;
+    std::normal_distribution<float> nd(0.f, 1.f);
+    std::vector<float> x((size_t)N * H * W * C), w((size_t)K * R * S * C), c((size_t)N * P * Q * K);
+    for (auto& v : x) v = nd(rng);
+    for (auto& v : w) v = nd(rng);
+    for (auto& v : c) v = nd(rng);
+    const int RSC = R * S * C, NPQ = N * P * Q;
+    std::vector<float> cols((size_t)RSC * NPQ, 0.f);
+    for (int r = 0; r < R; ++r)
+        for (int s = 0; s < S; ++s)
+            for (int ch = 0; ch < C; ++ch)
+                for (int b = 0; b < N; ++b)
+                    for (int p = 0; p < P; ++p)
+                        for (int q = 0; q < Q; ++q) {
+                            const int ih = p * sh - ph + r * dh, iw = q * sw - pw + s * dw;
+                            if (ih < 0 || ih >= H || iw < 0 || iw >= W) continue;
+                            cols[(size_t)((r * S + s) * C + ch) * NPQ + (b * P + p) * Q + q] =
+                                x[(((size_t)b * H + ih) * W + iw) * C + ch];
+                        }
+    // C (NHWC, i.e. NPQ x K) as the GEMM's K x NPQ operand
+    std::vector<float> cT((size_t)K * NPQ);
+    for (int n = 0; n < NPQ; ++n)
+        for (int k = 0; k < K; ++k) cT[(size_t)k * NPQ + n] = c[(size_t)n * K + k];
+    orc_accum acc{};
+    acc.kind = 0;
+    acc.mant_bits = 9;
+    acc.chunk = 1;
+    std::vector<float> D((size_t)K * NPQ);
+    orc_gemm_f32(K, NPQ, RSC, w.data(), RSC, 0, cols.data(), NPQ, 0, D.data(), NPQ, 0.5f, &acc,
+                 nullptr, 0, 0, cT.data(), NPQ, -2.0f);
+    std::vector<float> want((size_t)NPQ * K);
+    for (int n = 0; n < NPQ; ++n)
+        for (int k = 0; k < K; ++k) want[(size_t)n * K + k] = D[(size_t)k * NPQ + n];
+    float *dx = dev_copy(x), *dw_ = dev_copy(w), *dc = dev_copy(c);
+    float* dy = dev_copy(std::vector<float>(want.size(), 0.f));
+    Conv2d<> conv(9);
+    const auto st = conv(Conv2dProblemSize(N, H, W, C, K, R, S, P, Q, ph, pw, sh, sw, dh, dw),
+                         dx, dw_, dc, dy, 0.5f, -2.0f);
+    (void)hipDeviceSynchronize();
+    EXPECT(st == Conv2d<>::Status::kSuccess, "Conv2d status");
+    compare(host_copy(dy, want.size()), want, "Conv2d NHWC/KRSC");
+    (void)hipFree(dx); (void)hipFree(dw_); (void)hipFree(dc); (void)hipFree(dy);
+}
+
 static void gpu_checks() {
     std::mt19937 rng(1234);
     const int n = 100003;
@@ -307,6 +366,7 @@ static void gpu_checks() {
     compare(host_copy(dm, n), want, "run_virtual_mx_round e2m1");
     (void)hipFree(d); (void)hipFree(dw); (void)hipFree(dout); (void)hipFree(din); (void)hipFree(dm);
 
+    conv_case(rng);
     pack_gpu_checks(rng);
     for (float beta : {0.0f, 0.75f}) {
         gemm_case<RowMajor, RowMajor, RowMajor>(rng, beta);
