@@ -349,6 +349,28 @@ LOF_API lof_status lof_pack_reps(uint8_t* dst, const uint32_t* reps, int64_t cou
 LOF_API lof_status lof_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t count,
                                    int32_t bitwidth, int32_t endian, lof_stream_t stream);
 
+/* ---------------------------------------------------------------------------------
+ * Typed cast: FP32 <-> arrays of n-bit reps, one per container (uint8 for n <= 8, uint16
+ * little-endian for 9 <= n <= 16), i.e. the storage of a Templated_Float<Fp> array.  The
+ * conversion is the scalar ConvertImpl<float, Templated_Float<Fp>>::run of lo_float.h
+ * (lo_float.h:3057-3364; the runtime twin the codec restates), with the Saturation_Mode of
+ * spec, not the SIMD array run<arch> (lo_float.h:3497+), which drops the special values.
+ * ------------------------------------------------------------------------------- */
+/* Bytes per rep container: 1 (n <= 8), 2 (n <= 16), -1 outside [1, 16]. */
+LOF_API int32_t lof_rep_bytes(int32_t bitwidth);
+
+/* dst[i] <- the rep of src[i] in fmt under spec (random words as lof_pack_encode_f32).
+ * Replaces lo_float::Project<Templated_Float<Fp>>(float*, Templated_Float<Fp>*, n, ps)
+ * (lo_float.h:4014-4022) and the scalar Round<T> / Project<T> (lo_float.h:3979-3988). */
+LOF_API lof_status lof_cast_to_rep_f32(void* dst, const float* src, int64_t count,
+                                       const lof_pack_format* fmt, lof_round_spec spec,
+                                       const uint32_t* rand_bits, uint64_t seed,
+                                       lof_stream_t stream);
+/* dst[i] <- the FP32 value of rep src[i] (ConvertImpl<Templated_Float<Fp>, float>::run,
+ * the codec's convert_rep_to_float). */
+LOF_API lof_status lof_cast_from_rep_f32(float* dst, const void* src, int64_t count,
+                                         const lof_pack_format* fmt, lof_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

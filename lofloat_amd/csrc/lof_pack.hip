@@ -46,6 +46,7 @@ struct PackArgs {
     int32_t xm;              // slot of element e within the tile: e ^ xm (Reverse: gc - 1)
     int32_t aligned;         // the packed buffer is 4-byte aligned
     int32_t vec;             // the element-side buffers (src / dstf / reps / bits) are 16-byte aligned
+    int32_t rep_bytes;       // typed cast: bytes per rep container (1 for n <= 8, else 2)
     PackDev P;
 };
 
@@ -229,6 +230,119 @@ __global__ __launch_bounds__(kPackTPB) void k_pack_decode(PackArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Typed cast: one rep per 1- or 2-byte container (the storage of a Templated_Float<Fp>
+// array), no bit packing.  A block covers kCastTile elements; thread t handles the quads
+// k * 256 + t (k < 4), so every load and store instruction is coalesced across lanes.
+constexpr int kCastTile = kPackTPB * 16;
+
+__device__ __forceinline__ void store_rep_quad(const PackArgs& a, int64_t e0, const uint32_t (&r)[4],
+                                               bool whole) {
+    if (whole && a.aligned) {
+        if (a.rep_bytes == 1)
+            *reinterpret_cast<uint32_t*>(a.dst + e0) = r[0] | (r[1] << 8) | (r[2] << 16) | (r[3] << 24);
+        else
+            *reinterpret_cast<uint2*>(a.dst + 2 * e0) = make_uint2(r[0] | (r[1] << 16), r[2] | (r[3] << 16));
+        return;
+    }
+    for (int i = 0; i < 4; ++i) {
+        if (e0 + i >= a.count) break;
+        if (a.rep_bytes == 1) {
+            a.dst[e0 + i] = (uint8_t)r[i];
+        } else {
+            a.dst[2 * (e0 + i)] = (uint8_t)r[i];
+            a.dst[2 * (e0 + i) + 1] = (uint8_t)(r[i] >> 8);
+        }
+    }
+}
+
+template <int MODE, int RSRC>
+__global__ __launch_bounds__(kPackTPB) void k_cast_to_rep(PackArgs a) {
+    const int tid = threadIdx.x;
+    const int64_t bbase = (int64_t)blockIdx.x * kCastTile;
+    const uint32_t* xin = reinterpret_cast<const uint32_t*>(a.src);
+    if (a.vec && bbase + kCastTile <= a.count) {
+        uint4 x[4], u[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            x[k] = reinterpret_cast<const uint4*>(xin + bbase)[k * kPackTPB + tid];
+            if constexpr (RSRC == 1) u[k] = reinterpret_cast<const uint4*>(a.bits + bbase)[k * kPackTPB + tid];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int64_t e0 = bbase + 4 * (k * kPackTPB + tid);
+            if constexpr (RSRC == 2)
+                u[k] = make_uint4(rand_u32(a.seed, (uint64_t)e0), rand_u32(a.seed, (uint64_t)e0 + 1),
+                                  rand_u32(a.seed, (uint64_t)e0 + 2), rand_u32(a.seed, (uint64_t)e0 + 3));
+            else if constexpr (RSRC == 0)
+                u[k] = make_uint4(0u, 0u, 0u, 0u);
+            const uint32_t r[4] = {elem_rep<MODE, RSRC, false>(a, x[k].x, u[k].x),
+                                   elem_rep<MODE, RSRC, false>(a, x[k].y, u[k].y),
+                                   elem_rep<MODE, RSRC, false>(a, x[k].z, u[k].z),
+                                   elem_rep<MODE, RSRC, false>(a, x[k].w, u[k].w)};
+            store_rep_quad(a, e0, r, true);
+        }
+    } else {
+        for (int k = 0; k < 4; ++k) {
+            const int64_t e0 = bbase + 4 * (k * kPackTPB + tid);
+            if (e0 >= a.count) break;
+            uint32_t r[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (e0 + i < a.count)
+                    r[i] = elem_rep<MODE, RSRC, false>(a, xin[e0 + i], pack_word_for<RSRC>(a, e0 + i));
+            store_rep_quad(a, e0, r, e0 + 4 <= a.count);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t load_rep(const PackArgs& a, int64_t e) {
+    return a.rep_bytes == 1 ? (uint32_t)a.psrc[e]
+                            : (uint32_t)a.psrc[2 * e] | ((uint32_t)a.psrc[2 * e + 1] << 8);
+}
+
+__global__ __launch_bounds__(kPackTPB) void k_cast_from_rep(PackArgs a) {
+    const int tid = threadIdx.x;
+    const int64_t bbase = (int64_t)blockIdx.x * kCastTile;
+    uint32_t* out = reinterpret_cast<uint32_t*>(a.dstf);
+    if (a.vec && a.aligned && bbase + kCastTile <= a.count) {
+        uint32_t w[4][2];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int q = k * kPackTPB + tid;
+            if (a.rep_bytes == 1) {
+                w[k][0] = reinterpret_cast<const uint32_t*>(a.psrc + bbase)[q];
+                w[k][1] = 0u;
+            } else {
+                const uint2 v = reinterpret_cast<const uint2*>(a.psrc + 2 * bbase)[q];
+                w[k][0] = v.x;
+                w[k][1] = v.y;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t r[4];
+            if (a.rep_bytes == 1) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) r[i] = (w[k][0] >> (8 * i)) & 0xFFu;
+            } else {
+                r[0] = w[k][0] & 0xFFFFu; r[1] = w[k][0] >> 16;
+                r[2] = w[k][1] & 0xFFFFu; r[3] = w[k][1] >> 16;
+            }
+            reinterpret_cast<uint4*>(out + bbase)[k * kPackTPB + tid] =
+                make_uint4(__float_as_uint(rep_to_float(r[0], a.P)), __float_as_uint(rep_to_float(r[1], a.P)),
+                           __float_as_uint(rep_to_float(r[2], a.P)), __float_as_uint(rep_to_float(r[3], a.P)));
+        }
+    } else {
+        for (int k = 0; k < 4; ++k) {
+            const int64_t e0 = bbase + 4 * (k * kPackTPB + tid);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (e0 + i < a.count) out[e0 + i] = __float_as_uint(rep_to_float(load_rep(a, e0 + i), a.P));
+        }
+    }
+}
+
 int gcd_i(int a, int b) { return b == 0 ? a : gcd_i(b, a % b); }
 
 void geometry(int n, int* gc, int* gbits) {
@@ -332,6 +446,26 @@ dim3 grid_for(const PackArgs& a) { return dim3((unsigned)a.tiles); }
 template <int MODE, int RSRC>
 void launch_enc(const PackArgs& a, hipStream_t st) {
     hipLaunchKernelGGL((k_pack_encode<MODE, RSRC, false>), grid_for(a), dim3(kPackTPB), 0, st, a);
+}
+
+template <int MODE, int RSRC>
+void launch_cast(const PackArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL((k_cast_to_rep<MODE, RSRC>),
+                       dim3((unsigned)((a.count + kCastTile - 1) / kCastTile)), dim3(kPackTPB), 0,
+                       st, a);
+}
+
+lof_status launch_cast_to_rep(const PackArgs& a, int mode, int rsrc, hipStream_t st) {
+#define LOF_PC(M, R) if (mode == M && rsrc == R) { launch_cast<M, R>(a, st); goto done; }
+    LOF_PC(RNE, 0) LOF_PC(RTZ, 0) LOF_PC(RAFZ, 0) LOF_PC(RNO, 0) LOF_PC(RDOWN, 0)
+    LOF_PC(RUP, 0) LOF_PC(RTA, 0) LOF_PC(RTO, 0) LOF_PC(RTTZ, 0)
+    LOF_PC(SRA, 0) LOF_PC(SRB, 0) LOF_PC(SRC, 0) LOF_PC(SRD, 0)
+    LOF_PC(SRA, 1) LOF_PC(SRB, 1) LOF_PC(SRC, 1) LOF_PC(TRUESR, 1) LOF_PC(PROB, 1) LOF_PC(SRD, 1)
+    LOF_PC(SRA, 2) LOF_PC(SRB, 2) LOF_PC(SRC, 2) LOF_PC(TRUESR, 2) LOF_PC(PROB, 2) LOF_PC(SRD, 2)
+#undef LOF_PC
+    return set_error(LOF_ERR_UNSUPPORTED, "lof_cast_to_rep_f32: mode %d not supported", mode);
+done:
+    return check_hip(hipGetLastError(), "lof_cast_to_rep_f32 launch");
 }
 
 lof_status launch_encode(const PackArgs& a, int mode, int rsrc, hipStream_t st) {
@@ -453,6 +587,59 @@ LOF_API lof_status lof_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t c
     hipLaunchKernelGGL((k_pack_decode<true>), grid_for(a), dim3(kPackTPB), 0,
                        (hipStream_t)stream, a);
     return check_hip(hipGetLastError(), "lof_unpack_reps launch");
+}
+
+LOF_API int32_t lof_rep_bytes(int32_t bitwidth) {
+    return bitwidth < 1 || bitwidth > 16 ? -1 : (bitwidth <= 8 ? 1 : 2);
+}
+
+LOF_API lof_status lof_cast_to_rep_f32(void* dst, const float* src, int64_t count,
+                                       const lof_pack_format* fmt, lof_round_spec spec,
+                                       const uint32_t* rand_bits, uint64_t seed,
+                                       lof_stream_t stream) {
+    const char* what = "lof_cast_to_rep_f32";
+    PackArgs a{};
+    lof_status s = make_pack_dev(fmt, spec, &a.P, what);
+    if (s != LOF_OK) return s;
+    if (count < 0) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: count < 0", what);
+    if (count == 0) return LOF_OK;
+    if (!dst || !src) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: NULL buffer", what);
+    if ((count + kCastTile - 1) / kCastTile >= ((int64_t)1 << 31))
+        return set_error(LOF_ERR_UNSUPPORTED, "%s: input too large", what);
+    int rsrc = 0;
+    if (is_stochastic(spec.mode)) rsrc = rand_bits ? 1 : 2;
+    if (uses_len(spec.mode) && spec.stoch_len == 0) rsrc = 0;   // degenerate: no draw used
+    a.count = count;
+    a.src = src;
+    a.dst = static_cast<uint8_t*>(dst);
+    a.bits = rand_bits;
+    a.seed = seed;
+    a.rep_bytes = a.P.n <= 8 ? 1 : 2;
+    a.aligned = ((uintptr_t)dst & (uintptr_t)(4 * a.rep_bytes - 1)) == 0 ? 1 : 0;
+    a.vec = aligned16(src) && (rsrc != 1 || aligned16(rand_bits)) ? 1 : 0;
+    return launch_cast_to_rep(a, spec.mode, rsrc, (hipStream_t)stream);
+}
+
+LOF_API lof_status lof_cast_from_rep_f32(float* dst, const void* src, int64_t count,
+                                         const lof_pack_format* fmt, lof_stream_t stream) {
+    const char* what = "lof_cast_from_rep_f32";
+    PackArgs a{};
+    lof_status s = make_pack_dev(fmt, lof_round_spec{0, 0, 0}, &a.P, what);
+    if (s != LOF_OK) return s;
+    if (count < 0) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: count < 0", what);
+    if (count == 0) return LOF_OK;
+    if (!dst || !src) return set_error(LOF_ERR_INVALID_ARGUMENT, "%s: NULL buffer", what);
+    if ((count + kCastTile - 1) / kCastTile >= ((int64_t)1 << 31))
+        return set_error(LOF_ERR_UNSUPPORTED, "%s: input too large", what);
+    a.count = count;
+    a.dstf = dst;
+    a.psrc = static_cast<const uint8_t*>(src);
+    a.rep_bytes = a.P.n <= 8 ? 1 : 2;
+    a.aligned = ((uintptr_t)src & (uintptr_t)(4 * a.rep_bytes - 1)) == 0 ? 1 : 0;
+    a.vec = aligned16(dst) ? 1 : 0;
+    hipLaunchKernelGGL(k_cast_from_rep, dim3((unsigned)((count + kCastTile - 1) / kCastTile)),
+                       dim3(kPackTPB), 0, (hipStream_t)stream, a);
+    return check_hip(hipGetLastError(), "lof_cast_from_rep_f32 launch");
 }
 
 }  // extern "C"

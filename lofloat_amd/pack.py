@@ -18,7 +18,8 @@ from . import _lib
 from .core import FloatFormatDescriptor, RoundingMode
 from .ops import _bits_ptr, _mode, _require_device_f32, _seed_for, _stream
 
-__all__ = ["packed_size", "encode", "decode", "pack_reps", "unpack_reps", "pack_format"]
+__all__ = ["packed_size", "encode", "decode", "pack_reps", "unpack_reps", "pack_format",
+           "rep_dtype", "to_reps", "from_reps"]
 
 _ENDIAN = {"forward": 0, "reverse": 1, 0: 0, 1: 1}
 
@@ -125,4 +126,62 @@ def unpack_reps(buf: torch.Tensor, bitwidth: int, count: int, endian="forward") 
                                       ctypes.c_void_p(buf.data_ptr()), int(count),
                                       int(bitwidth), _ENDIAN[endian], _stream(buf))
     _lib.check(st, "pack.unpack_reps")
+    return y
+
+
+# ---------------------------------------------------------------------------- typed cast
+def rep_dtype(bitwidth: int) -> torch.dtype:
+    """Container of one n-bit rep: uint8 for n <= 8, int16 (the uint16 bits) for n <= 16."""
+    b = int(_lib.lib.lof_rep_bytes(int(bitwidth)))
+    if b < 0:
+        raise NotImplementedError(f"typed cast: bitwidth {bitwidth} outside [1, 16]")
+    return torch.uint8 if b == 1 else torch.int16
+
+
+def to_reps(input: torch.Tensor, params: FloatFormatDescriptor,
+            round_mode=RoundingMode.RoundToNearestEven, stoch_len: int = 0, saturation: int = 0,
+            *, unsigned_behavior: int = 0, rand_bits=None, seed=None,
+            out: torch.Tensor | None = None) -> torch.Tensor:
+    """The typed cast lo_float::Project<Templated_Float<Fp>>(x, y, n, ps) (lo_float.h:
+    4014-4022, with the scalar ConvertImpl semantics of lo_float.h:3057-3364): each element
+    rounded to ``params`` and stored as its rep, one per uint8 (n <= 8) or int16 (n <= 16)
+    element, same shape as the input.  ``saturation``: 0 OvfInf, 1 SatFinite,
+    2 SatPropagate."""
+    _require_device_f32(input, "to_reps")
+    x = input.contiguous()
+    pf = pack_format(params, unsigned_behavior)
+    dt = rep_dtype(params.total_bits)
+    y = torch.empty(x.shape, dtype=dt, device=x.device) if out is None else out
+    if y.dtype != dt or y.shape != x.shape or not y.is_contiguous() or y.device != x.device:
+        raise ValueError(f"to_reps: out must be a contiguous {dt} tensor of shape {tuple(x.shape)}")
+    mode = _mode(round_mode)
+    bptr, keep = _bits_ptr(rand_bits, x, "to_reps")
+    with torch.cuda.device(x.device):
+        st = _lib.lib.lof_cast_to_rep_f32(
+            ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), x.numel(),
+            ctypes.byref(pf), _lib.RoundSpec(mode, int(saturation), int(stoch_len)), bptr,
+            _seed_for(mode, seed), _stream(x))
+    _lib.check(st, "to_reps")
+    del keep
+    return y
+
+
+def from_reps(reps: torch.Tensor, params: FloatFormatDescriptor, *, unsigned_behavior: int = 0,
+              out: torch.Tensor | None = None) -> torch.Tensor:
+    """The FP32 value of each rep (ConvertImpl<Templated_Float<Fp>, float>::run); ``reps``
+    is a uint8 (n <= 8) or int16 (n <= 16) tensor as to_reps returns."""
+    dt = rep_dtype(params.total_bits)
+    if not reps.is_cuda or reps.dtype != dt:
+        raise RuntimeError(f"from_reps: reps must be a {dt} device tensor for a "
+                           f"{params.total_bits}-bit format")
+    r = reps.contiguous()
+    pf = pack_format(params, unsigned_behavior)
+    y = torch.empty(r.shape, dtype=torch.float32, device=r.device) if out is None else out
+    if y.dtype != torch.float32 or y.shape != r.shape or not y.is_contiguous():
+        raise ValueError("from_reps: out must be a contiguous float32 tensor of reps' shape")
+    with torch.cuda.device(r.device):
+        st = _lib.lib.lof_cast_from_rep_f32(ctypes.c_void_p(y.data_ptr()),
+                                            ctypes.c_void_p(r.data_ptr()), r.numel(),
+                                            ctypes.byref(pf), _stream(r))
+    _lib.check(st, "from_reps")
     return y

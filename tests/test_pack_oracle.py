@@ -350,3 +350,46 @@ def test_stochastic_encode_unbiased_below_normal():
         dec = oracle.rep_to_float(oracle.float_to_rep(x, pf, oracle.TRUESR, seed=5), pf)
         sd = 2.0 ** -9 / 2 / math.sqrt(x.size)
         assert abs(dec.astype(np.float64).mean() - float(np.float32(v))) < 5 * sd, v
+
+
+def _p3109_max(l, p, ext):
+    """Largest finite value of signed P3109<l, p> from its layout: the top code below the
+    sign bit, minus one when it is Inf (Extended); bias 2^(l-p-1), p-1 mantissa bits."""
+    code = (1 << (l - 1)) - 1 - (1 if ext else 0)
+    mb, bias = p - 1, 1 << (l - p - 1)
+    E, M = code >> mb, code & ((1 << mb) - 1)
+    if E == 0:
+        return math.ldexp(M, 1 - bias - mb)
+    return math.ldexp((1 << mb) + M, E - bias - mb)
+
+
+@pytest.mark.parametrize("l", range(2, 9))
+def test_saturation_modes_sweep(l):
+    """The reference's test_sat_mode.cpp sweep (test_sat_mode.cpp:38-150) on the typed cast:
+    every P3109<l, p>, p = 1..l-1, Saturating and Extended.  Finite overflow: SatFinite and
+    SatPropagate give +-max, OvfInf +-inf where the format has a real infinity (else +-max);
+    +-inf input: SatFinite +-max, OvfInf / SatPropagate +-inf (else +-max); 1.0 is the same
+    under all three modes."""
+    for p in range(1, l):
+        for ext in (False, True):
+            pf = _pf((l, p - 1, 1 << (l - p - 1), 0 if ext else 1, 0, 0), ("p3109", l, True, ext))
+            maxf = _p3109_max(l, p, ext)
+            if not maxf > 0.0:
+                continue   # degenerate (e.g. <2,1> Extended): no finite nonzero value
+            inf_code = (1 << (l - 1)) - 1
+            real_inf = ext and math.isinf(float(oracle.rep_to_float([inf_code], pf)[0]))
+            dec = lambda x, sat: float(oracle.rep_to_float(      # noqa: E731
+                oracle.float_to_rep(np.array([x], np.float32), pf, oracle.RNE, sat), pf)[0])
+            for m in (2.0, 8.0, 137.0, 1e6):
+                for sign in (1.0, -1.0):
+                    x = sign * maxf * m
+                    if not math.isfinite(np.float32(x)):
+                        continue
+                    assert dec(x, 1) == sign * maxf and dec(x, 2) == sign * maxf, (l, p, ext, x)
+                    assert dec(x, 0) == (sign * math.inf if real_inf else sign * maxf), (l, p, ext, x)
+            for sign in (1.0, -1.0):
+                assert dec(sign * math.inf, 1) == sign * maxf
+                want = sign * math.inf if real_inf else sign * maxf
+                assert dec(sign * math.inf, 0) == want and dec(sign * math.inf, 2) == want
+            r = [dec(1.0, s) for s in (0, 1, 2)]
+            assert r[0] == r[1] == r[2] and math.isfinite(r[0])
