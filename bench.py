@@ -326,7 +326,15 @@ def pack_codec(lof, torch, np, dev, out):
     out["unpack_reps_2^28_n11_GBs"] = round(algo / ms_d / 1e6, 1)
     log(f"pack_reps n=11: {ms * 1e3:.0f} us, unpack {ms_d * 1e3:.0f} us")
     del reps, buf
-    del x, y
+    # typed cast (one rep per byte): x -> e4m3fn reps and back, 4 + 1 B per element
+    e4 = lof.create_ocp_e4m3fn_params()
+    r8 = torch.empty(n, dtype=torch.uint8, device=dev)
+    ms = timed(lambda i: lof.pack.to_reps(x, e4, out=r8), 10)
+    ms_d = timed(lambda i: lof.pack.from_reps(r8, e4, out=y), 10)
+    out["cast_to_reps_2^28_e4m3fn_GBs"] = round(5 * n / ms / 1e6, 1)
+    out["cast_from_reps_2^28_e4m3fn_GBs"] = round(5 * n / ms_d / 1e6, 1)
+    log(f"typed cast e4m3fn: to {ms * 1e3:.0f} us, from {ms_d * 1e3:.0f} us")
+    del x, y, r8
     torch.cuda.empty_cache()
 
 

@@ -8,6 +8,7 @@
 //   Layout, Matrix<T, idx, Layout>                src/layouts.h:16-19, src/Matrix.h:21-78
 //   Gemm<MatA, MatB, MatC, MatD>                  src/cutlass_gemms.cuh:1629-1896
 //   Conv2d<...>, Conv2dProblemSize                src/cutlass_conv.cuh:20-184
+//   Project over rep arrays (typed cast)          src/lo_float.h:3979-3988, 4014-4022
 //   PackEndian, LoF_packed_size, LoF_encode_cuda, LoF_decode_cuda and the checker
 //   functors they read                           src/lo_pack.hpp:50, 392-531; lo_float.h:1411-1789
 // Same names, argument meanings and enum values.  Differences: every launcher takes an
@@ -565,6 +566,36 @@ inline lof_pack_format to_pack(const FloatingPointParams<I, N>& p) {
                            checker_of<N>()};
 }
 }  // namespace detail
+
+// Typed cast (lo_float.h:3979-3988, 4014-4022): Project over device arrays of reps, one per
+// uint8_t (formats of <= 8 bits) or uint16_t (<= 16 bits) -- the storage of a
+// Templated_Float<Fp> array -- with the scalar ConvertImpl semantics and ps's saturation
+// mode.  The rep's low n bits are stored.
+template <class I, class N, class Rep>
+void Project(const float* d_x, Rep* d_y, int n, FloatingPointParams<I, N> params,
+             ProjSpec ps = ProjSpec{}, hipStream_t stream = nullptr,
+             const uint32_t* rand_bits = nullptr, uint64_t seed = 0) {
+    static_assert(std::is_same_v<Rep, uint8_t> || std::is_same_v<Rep, uint16_t>,
+                  "reps are stored in uint8_t or uint16_t");
+    if (lof_rep_bytes(params.bitwidth) != (int32_t)sizeof(Rep))
+        throw Error(LOF_ERR_INVALID_ARGUMENT, "Project: container size does not fit the format");
+    const lof_pack_format f = detail::to_pack(params);
+    detail::check(lof_cast_to_rep_f32(d_y, d_x, n, &f, detail::to_c(ps), rand_bits, seed,
+                                      (lof_stream_t)stream),
+                  "Project");
+}
+
+// The FP32 values of device reps (ConvertImpl<Templated_Float<Fp>, float>::run).
+template <class I, class N, class Rep>
+void Project(const Rep* d_x, float* d_y, int n, FloatingPointParams<I, N> params,
+             hipStream_t stream = nullptr) {
+    static_assert(std::is_same_v<Rep, uint8_t> || std::is_same_v<Rep, uint16_t>,
+                  "reps are stored in uint8_t or uint16_t");
+    if (lof_rep_bytes(params.bitwidth) != (int32_t)sizeof(Rep))
+        throw Error(LOF_ERR_INVALID_ARGUMENT, "Project: container size does not fit the format");
+    const lof_pack_format f = detail::to_pack(params);
+    detail::check(lof_cast_from_rep_f32(d_y, d_x, n, &f, (lof_stream_t)stream), "Project");
+}
 
 // Bytes of a packed buffer of `count` n-bit elements (lo_pack.hpp:392-394).
 inline size_t LoF_packed_size(int n, size_t count) {

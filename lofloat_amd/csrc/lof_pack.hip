@@ -518,8 +518,16 @@ LOF_API lof_status lof_pack_encode_f32(uint8_t* dst, const float* src, int64_t c
     a.dst = dst;
     a.bits = rand_bits;
     a.seed = seed;
-    a.aligned = ((uintptr_t)dst & 3u) == 0 ? 1 : 0;
     a.vec = aligned16(src) && (rsrc != 1 || aligned16(rand_bits)) ? 1 : 0;
+    if (a.P.n == 8 || a.P.n == 16) {
+        // one element per group of whole bytes (either order): the packed stream is the
+        // little-endian rep containers, written directly without the LDS slot stage
+        if (count == 0) return LOF_OK;
+        a.rep_bytes = a.P.n / 8;
+        a.aligned = ((uintptr_t)dst & (uintptr_t)(4 * a.rep_bytes - 1)) == 0 ? 1 : 0;
+        return launch_cast_to_rep(a, spec.mode, rsrc, (hipStream_t)stream);
+    }
+    a.aligned = ((uintptr_t)dst & 3u) == 0 ? 1 : 0;
     return launch_encode(a, spec.mode, rsrc, (hipStream_t)stream);
 }
 
@@ -538,8 +546,15 @@ LOF_API lof_status lof_pack_decode_f32(float* dst, const uint8_t* src, int64_t c
     if (a.tiles == 0) return LOF_OK;
     a.dstf = dst;
     a.psrc = src;
-    a.aligned = ((uintptr_t)src & 3u) == 0 ? 1 : 0;
     a.vec = aligned16(dst) ? 1 : 0;
+    if (a.P.n == 8 || a.P.n == 16) {   // byte-aligned single-element groups: rep containers
+        a.rep_bytes = a.P.n / 8;
+        a.aligned = ((uintptr_t)src & (uintptr_t)(4 * a.rep_bytes - 1)) == 0 ? 1 : 0;
+        hipLaunchKernelGGL(k_cast_from_rep, dim3((unsigned)((count + kCastTile - 1) / kCastTile)),
+                           dim3(kPackTPB), 0, (hipStream_t)stream, a);
+        return check_hip(hipGetLastError(), "lof_pack_decode_f32 launch");
+    }
+    a.aligned = ((uintptr_t)src & 3u) == 0 ? 1 : 0;
     hipLaunchKernelGGL((k_pack_decode<false>), grid_for(a), dim3(kPackTPB), 0,
                        (hipStream_t)stream, a);
     return check_hip(hipGetLastError(), "lof_pack_decode_f32 launch");

@@ -323,6 +323,51 @@ static void conv_case(std::mt19937& rng) {
     (void)hipFree(dx); (void)hipFree(dw_); (void)hipFree(dc); (void)hipFree(dy);
 }
 
+// Project (typed cast) both ways against orc_float_to_rep_array / orc_rep_to_float_array.
+static void project_case(std::mt19937& rng) {
+    std::vector<float> x(10007);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    for (auto& v : x) v = nd(rng) * 64.f;
+    x[0] = INFINITY; x[1] = -INFINITY; x[2] = NAN; x[3] = 1e30f;
+    const FloatingPointParams e5m2(8, 2, 15, Saturating, _3109, Signed, OCP_F8E4M3_InfChecker{},
+                                   OCP_F8E5M2_NaNChecker{});
+    const FloatingPointParams half(16, 10, 15, Extended, _3109, Signed, FP16InfChecker{},
+                                   FP16NaNChecker{});
+    float* dx = dev_copy(x);
+    const auto run = [&](auto params, auto zero, ProjSpec ps, const char* what) {
+        using Rep = decltype(zero);
+        Rep* dr = dev_copy(std::vector<Rep>(x.size(), 0));
+        Project(dx, dr, (int)x.size(), params, ps);
+        float* dy = dev_copy(std::vector<float>(x.size(), 0.f));
+        Project(dr, dy, (int)x.size(), params);
+        (void)hipDeviceSynchronize();
+        const orc_pack_format of = to_orc(detail::to_pack(params));
+        std::vector<uint32_t> want(x.size());
+        orc_float_to_rep_array(x.data(), want.data(), (int64_t)x.size(), &of,
+                               orc_spec{(int32_t)ps.rounding_mode, (int32_t)ps.saturation_mode,
+                                        ps.stoch_length}, nullptr, 0);
+        const std::vector<Rep> got = host_copy(dr, x.size());
+        size_t bad = 0;
+        for (size_t i = 0; i < x.size(); ++i)
+            bad += got[i] != (Rep)(want[i] & ((1u << params.bitwidth) - 1u));
+        EXPECT(bad == 0, "%s: %zu reps differ", what, bad);
+        std::vector<uint32_t> masked(x.size());
+        for (size_t i = 0; i < x.size(); ++i) masked[i] = want[i] & ((1u << params.bitwidth) - 1u);
+        std::vector<float> vals(x.size());
+        orc_rep_to_float_array(masked.data(), vals.data(), (int64_t)x.size(), &of);
+        compare(host_copy(dy, x.size()), vals, what);
+        (void)hipFree(dr); (void)hipFree(dy);
+    };
+    run(e5m2, uint8_t{0}, ProjSpec(Rounding_Mode::RoundToNearestEven, SatFinite), "Project e5m2");
+    run(half, uint16_t{0}, ProjSpec(Rounding_Mode::RoundTowardsZero, SatPropagate), "Project half");
+    try {
+        Project(dx, (uint16_t*)nullptr, 1, e5m2);
+        EXPECT(false, "an 8-bit format in uint16_t containers must throw");
+    } catch (const Error&) {
+    }
+    (void)hipFree(dx);
+}
+
 static void gpu_checks() {
     std::mt19937 rng(1234);
     const int n = 100003;
@@ -367,6 +412,7 @@ static void gpu_checks() {
     (void)hipFree(d); (void)hipFree(dw); (void)hipFree(dout); (void)hipFree(din); (void)hipFree(dm);
 
     conv_case(rng);
+    project_case(rng);
     pack_gpu_checks(rng);
     for (float beta : {0.0f, 0.75f}) {
         gemm_case<RowMajor, RowMajor, RowMajor>(rng, beta);
