@@ -350,6 +350,19 @@ LOF_API lof_status lof_unpack_reps(uint32_t* reps, const uint8_t* src, int64_t c
                                    int32_t bitwidth, int32_t endian, lof_stream_t stream);
 
 /* ---------------------------------------------------------------------------------
+ * Piecewise-linear SiLU: y = linear interpolation of an (N+1)-knot table over [-R, R]
+ * for |x| <= R, relu(x) outside, computed in float for every element type.
+ * ------------------------------------------------------------------------------- */
+enum { LOF_DTYPE_F32 = 0, LOF_DTYPE_F16 = 1, LOF_DTYPE_F64 = 2 };
+
+/* out[i] = pwl(in[i]) with lut[0..segments] on the device; segments a power of two in
+ * [2, 4096]; in / out of element type dtype (they may alias).  Replaces pwl_silu_kernel /
+ * pwl_silu (Lof_kernel.cu:231-265) and pwl_silu_forward (LoPy_bind.cpp:228-277). */
+LOF_API lof_status lof_pwl_silu(void* out, const void* in, int64_t n, int32_t dtype,
+                                const float* lut, int32_t segments, float R,
+                                lof_stream_t stream);
+
+/* ---------------------------------------------------------------------------------
  * Typed cast: FP32 <-> arrays of n-bit reps, one per container (uint8 for n <= 8, uint16
  * little-endian for 9 <= n <= 16), i.e. the storage of a Templated_Float<Fp> array.  The
  * conversion is the scalar ConvertImpl<float, Templated_Float<Fp>>::run of lo_float.h

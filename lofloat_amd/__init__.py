@@ -1,7 +1,8 @@
 """lofloat_amd -- MI355X (gfx950) implementation of LoFloat's data-parallel hot path.
 
 Drop-in names of the reference package ``LoFloat`` (LoFloat/__init__.py:1-5):
-``virtual_round``, ``virtual_mx_round``, ``lof_gemm``, ``lof_conv2d``, ``RoundingMode``, ``InfBehavior``,
+``virtual_round``, ``virtual_mx_round``, ``lof_gemm``, ``lof_conv2d``, ``pwl_silu``,
+``RoundingMode``, ``InfBehavior``,
 ``NaNBehavior``, ``Signedness``, ``FloatFormatDescriptor``, the ``create_*_params``
 factories, the ``torch.ops.lofloat.*`` operators and the layers; ``lofloat_amd.pack`` is
 the device bit-pack codec of lo_pack.hpp.  The arithmetic runs in
@@ -17,7 +18,7 @@ from .formats import (create_bfloat16_params, create_e8m0_params,  # noqa: F401
                       create_single_params, make_format)
 from .ops import (AccumSpec, conv2d, fill_random_u32, gemm, gemm_autotune,  # noqa: F401
                   gemm_config_name, gemm_config_valid, gemm_num_configs, lof_conv2d,
-                  lof_gemm, set_seed, virtual_mx_round, virtual_round)
+                  lof_gemm, pwl_silu, set_seed, virtual_mx_round, virtual_round)
 from . import _custom_ops  # noqa: F401,E402  (registers torch.ops.lofloat.*)
 from ._custom_ops import get_format, register_format  # noqa: F401,E402
 from .layers import (ExplicitMultiheadAttention, FISRBatchNorm, L1BatchNorm,  # noqa: F401,E402
@@ -30,15 +31,3 @@ from .grouped import GemmJob, GroupedPlan, MXJob, RoundJob  # noqa: F401,E402
 
 __version__ = "0.1.0"
 
-
-def _out_of_scope(name: str):
-    def f(*args, **kwargs):
-        raise NotImplementedError(
-            f"lofloat_amd.{name}: not part of the accelerated hot path (virtual rounding, MX "
-            "block rounding, emulated-accumulation GEMM, LoF_Linear); use the reference "
-            "LoFloat package for it")
-    f.__name__ = name
-    return f
-
-
-pwl_silu = _out_of_scope("pwl_silu")

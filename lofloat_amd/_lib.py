@@ -99,6 +99,7 @@ SIGNATURES = {
     "lof_pack_reps": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp]),
     "lof_unpack_reps": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp]),
     "lof_rep_bytes": (_i32, [_i32]),
+    "lof_pwl_silu": (_i32, [_vp, _vp, _i64, _i32, _vp, _i32, ctypes.c_float, _vp]),
     "lof_cast_to_rep_f32": (_i32, [_vp, _vp, _i64, _P(PackFormat), RoundSpec, _vp, _u64, _vp]),
     "lof_cast_from_rep_f32": (_i32, [_vp, _vp, _i64, _P(PackFormat), _vp]),
     "lof_grouped_run": (_i32, [_vp, _vp]),

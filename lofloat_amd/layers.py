@@ -586,6 +586,83 @@ class LoF_Conv2d(_FormatSetters, nn.Module):
                     f"backend={self.backend}")
 
 
+class _PWLSiLUApprox(torch.autograd.Function):
+    """pwl_silu forward; backward with the analytic SiLU derivative
+    sigmoid(x) * (1 + x * (1 - sigmoid(x))) as the surrogate (layers.py:1050-1069): the
+    piecewise-linear forward's own derivative is the segment slope."""
+
+    @staticmethod
+    def forward(ctx, x, lut, R):
+        ctx.save_for_backward(x)
+        return ops.pwl_silu(x.detach(), lut, float(R))
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        (x,) = ctx.saved_tensors
+        s = torch.sigmoid(x)
+        return grad_output * (s * (1 + x * (1 - s))), None, None
+
+
+class PWLSiLU(nn.Module):
+    """SiLU as a piecewise-linear table of 2^lut_bits uniform segments on [-R, R], the
+    asymptotes 0 / x outside (layers.py:1072-1137).  The table's knot values follow
+    ``lut_method``: "uniform" (SiLU at the knots), "midpoint" (SiLU at the segment
+    midpoints for the interior knots) or "minimax" (knot values shifted by the average of
+    the adjacent segments' midpoint errors, halving the maximum error of the chord).
+    ``use_ext=False`` runs the torch op chain the kernel replaces."""
+
+    def __init__(self, R=8.0, lut_bits=4, lut_method="minimax", use_ext=True):
+        super().__init__()
+        self.R = float(R)
+        self.lut_bits = lut_bits
+        self.lut_size = 1 << lut_bits
+        if self.lut_size % 2 != 0:
+            raise ValueError("lut_size must be even so x=0 is a knot.")
+        self.use_ext = use_ext and self.lut_size <= 4096
+        self.register_buffer("lut", torch.empty(self.lut_size + 1))
+        self.init_lut(lut_method)
+
+    @staticmethod
+    def _silu(x):
+        return x * torch.sigmoid(x)
+
+    def init_lut(self, method="minimax"):
+        N = self.lut_size
+        knots = torch.linspace(-self.R, self.R, N + 1, dtype=self.lut.dtype)
+        at_knots = self._silu(knots)
+        mids = 0.5 * (knots[:-1] + knots[1:])
+        if method == "uniform":
+            lut = at_knots.clone()
+        elif method == "midpoint":
+            lut = at_knots.clone()
+            lut[1:-1] = self._silu(mids)[:-1]
+        elif method == "minimax":
+            err = self._silu(mids) - 0.5 * (at_knots[:-1] + at_knots[1:])   # chord error
+            shift = torch.empty_like(at_knots)
+            shift[0] = err[0] / 2
+            shift[1:-1] = (err[:-1] + err[1:]) / 4
+            shift[-1] = err[-1] / 2
+            lut = at_knots + shift
+        else:
+            raise ValueError(f"Unknown lut_method {method!r}")
+        with torch.no_grad():
+            self.lut.copy_(lut)
+        self.lut_method = method
+
+    def forward(self, x):
+        if self.use_ext:
+            return _PWLSiLUApprox.apply(x, self.lut, self.R)
+        return self._forward_ref(x)
+
+    def _forward_ref(self, x):
+        N, R = self.lut_size, self.R
+        u = (x.clamp(-R, R) + R) * (N / (2 * R))
+        idx = u.long().clamp_(max=N - 1)
+        frac = u - idx.to(x.dtype)
+        out = torch.lerp(self.lut[idx], self.lut[idx + 1], frac)
+        return torch.where(x.abs() > R, torch.relu(x), out)
+
+
 def _not_in_scope(name):
     class _Missing(nn.Module):
         def __init__(self, *a, **k):
@@ -601,7 +678,6 @@ L1BatchNorm = _not_in_scope("L1BatchNorm")
 LinfBatchNorm = _not_in_scope("LinfBatchNorm")
 FISRBatchNorm = _not_in_scope("FISRBatchNorm")
 PWLBatchNorm = _not_in_scope("PWLBatchNorm")
-PWLSiLU = _not_in_scope("PWLSiLU")
 
 
 def replace_mha_with_explicit(*args, **kwargs):

@@ -333,6 +333,34 @@ def lof_conv2d(input: torch.Tensor, weight: torch.Tensor, pad_h: int, pad_w: int
                   dilation=(dilation_h, dilation_w), alpha=alpha)
 
 
+_SILU_DTYPES = {torch.float32: 0, torch.float16: 1, torch.float64: 2}
+
+
+def pwl_silu(input: torch.Tensor, lut: torch.Tensor, R: float) -> torch.Tensor:
+    """The reference's pwl_silu (LoPy_bind.cpp:228-277, Lof_kernel.cu:240-258): SiLU by
+    linear interpolation of the (N+1)-knot table ``lut`` over [-R, R] (N a power of two in
+    [2, 4096]), relu(x) outside, computed in float for float32 / float16 / float64 inputs;
+    returns a new tensor of the input's shape and dtype."""
+    if not isinstance(input, torch.Tensor) or input.device.type != "cuda":
+        raise RuntimeError("pwl_silu: lofloat_amd runs on HIP device tensors only")
+    if lut.dim() != 1 or lut.numel() < 2:
+        raise RuntimeError("pwl_silu: lut must be 1-D with >= 2 entries")
+    if not R > 0.0:
+        raise RuntimeError("pwl_silu: R must be positive")
+    dt = _SILU_DTYPES.get(input.dtype)
+    if dt is None:
+        raise RuntimeError(f"pwl_silu: dtype {input.dtype} not supported")
+    x = input.contiguous()
+    t = lut.to(device=x.device, dtype=torch.float32).contiguous()
+    y = torch.empty_like(x)
+    with torch.cuda.device(x.device):
+        st = _lib.lib.lof_pwl_silu(ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                   x.numel(), dt, ctypes.c_void_p(t.data_ptr()), t.numel() - 1,
+                                   float(R), _stream(x))
+    _lib.check(st, "pwl_silu")
+    return y
+
+
 def gemm_num_configs() -> int:
     return int(_lib.lib.lof_gemm_num_configs())
 

@@ -111,6 +111,8 @@ def _load() -> ctypes.CDLL:
     lib.orc_rep_to_float_array.restype = None
     lib.orc_rep_to_float_array.argtypes = [u32p, f32p, i64, P(_PackFmt)]
     lib.orc_max_threads.restype = i32
+    lib.orc_pwl_silu_array.restype = None
+    lib.orc_pwl_silu_array.argtypes = [f32p, f32p, i64, f32p, i32, ctypes.c_float]
     return lib
 
 
@@ -258,6 +260,18 @@ def gemm(A, B, accum: _Accum, trans_a: bool = False, trans_b: bool = False,
                       _ptr(B, ctypes.c_float), ldb, int(trans_b), _ptr(out, ctypes.c_float), N,
                       float(alpha), ctypes.byref(accum), None, 0, nthreads,
                       _ptr(Cc, ctypes.c_float), N, float(beta))
+    return out
+
+
+# ------------------------------------------------------------------ piecewise-linear SiLU
+def pwl_silu(x, lut, R: float) -> np.ndarray:
+    """pwl_silu_kernel (Lof_kernel.cu:240-258) elementwise on float32 x with an
+    (N+1)-entry float32 table over [-R, R]."""
+    x = _f32(x)
+    t = np.ascontiguousarray(lut, dtype=np.float32)
+    out = np.empty_like(x)
+    _lib.orc_pwl_silu_array(_ptr(x, ctypes.c_float), _ptr(out, ctypes.c_float), x.size,
+                            _ptr(t, ctypes.c_float), t.size - 1, float(np.float32(R)))
     return out
 
 

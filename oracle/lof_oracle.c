@@ -621,3 +621,25 @@ void orc_float_to_rep_array(const float* x, uint32_t* out, int64_t n, const orc_
 void orc_rep_to_float_array(const uint32_t* rep, float* out, int64_t n, const orc_pack_format* f) {
     for (int64_t i = 0; i < n; ++i) out[i] = orc_rep_to_float(rep[i], f);
 }
+
+/* ---------------------------------------------------------------- piecewise-linear SiLU */
+/* pwl_silu_kernel (Lof_kernel.cu:240-258) for one element, in float as the kernel computes:
+ * clamp to [-R, R] (fmaxf first, so NaN becomes -R), scale by inv_step = N / (2R) computed
+ * as in pwl_silu (Lof_kernel.cu:261), truncate to the segment index capped at N-1, fma
+ * interpolation, relu outside [-R, R]. */
+float orc_pwl_silu(float x, const float* lut, int N, float R) {
+    const float inv_step = (float)N / (2.0f * R);
+    const float xc = fminf(fmaxf(x, -R), R);
+    const float u = (xc + R) * inv_step;
+    int i = (int)u;
+    if (i > N - 1) i = N - 1;
+    const float frac = u - (float)i;
+    const float y0 = lut[i], y1 = lut[i + 1];
+    const float y = fmaf(frac, y1 - y0, y0);
+    if (fabsf(x) > R) return x > 0.0f ? x : 0.0f;
+    return y;
+}
+
+void orc_pwl_silu_array(const float* x, float* y, int64_t n, const float* lut, int N, float R) {
+    for (int64_t k = 0; k < n; ++k) y[k] = orc_pwl_silu(x[k], lut, N, R);
+}

@@ -144,6 +144,10 @@ void orc_rep_to_float_array(const uint32_t* rep, float* out, int64_t n, const or
 
 int orc_max_threads(void);
 
+/* pwl_silu_kernel (Lof_kernel.cu:240-258): one element, and an array. */
+float orc_pwl_silu(float x, const float* lut, int N, float R);
+void orc_pwl_silu_array(const float* x, float* y, int64_t n, const float* lut, int N, float R);
+
 #ifdef __cplusplus
 }
 #endif
