@@ -328,13 +328,17 @@ def pack_codec(lof, torch, np, dev, out):
     del reps, buf
     # typed cast (one rep per byte): x -> e4m3fn reps and back, 4 + 1 B per element
     e4 = lof.create_ocp_e4m3fn_params()
+    lut = lof.PWLSiLU(R=8.0, lut_bits=4).lut.to(dev)
     r8 = torch.empty(n, dtype=torch.uint8, device=dev)
     ms = timed(lambda i: lof.pack.to_reps(x, e4, out=r8), 10)
     ms_d = timed(lambda i: lof.pack.from_reps(r8, e4, out=y), 10)
+    ms_s = timed(lambda i: lof.pwl_silu(x, lut, 8.0), 10)
+    out["pwl_silu_2^28_f32_GBs"] = round(8 * n / ms_s / 1e6, 1)
+    log(f"pwl_silu 2^28 f32 (lut 16): {ms_s * 1e3:.0f} us ({8 * n / ms_s / 1e6:.0f} GB/s)")
     out["cast_to_reps_2^28_e4m3fn_GBs"] = round(5 * n / ms / 1e6, 1)
     out["cast_from_reps_2^28_e4m3fn_GBs"] = round(5 * n / ms_d / 1e6, 1)
     log(f"typed cast e4m3fn: to {ms * 1e3:.0f} us, from {ms_d * 1e3:.0f} us")
-    del x, y, r8
+    del x, y, r8, lut
     torch.cuda.empty_cache()
 
 
