@@ -2,9 +2,9 @@
 //
 // The plan uploads one GroupTask per job and the first tile index of each job (jobs are
 // ordered GEMMs first, largest first, then rounding jobs, largest first).  GEMMs with
-// k-chunk 8, 16 or 32 use the 64 x 64 MFMA chunk-sum tile, the others the config's VALU
-// tile.
-// A fixed grid of
+// k-chunk 8, 16 or 32 use the config's MFMA chunk-sum tile (64 x 64 or 128 x 64), the
+// others the config's VALU tile.  The kernel is held to two workgroups per CU (launch
+// bounds), which the 128 x 64 MFMA tile would otherwise drop to one.  A fixed grid of
 // workgroups (as many as fit on the device at once) repeatedly claims the next tile with
 // an atomic counter, finds its job by binary search over the start indices, and runs the
 // job's tile routine: the same device routines as the single-job kernels, dispatched once
@@ -131,7 +131,7 @@ __device__ __forceinline__ void grouped_mfma(const GemmArgs& g, int tm, int tn, 
 }
 
 template <class C>
-__global__ __launch_bounds__(256) void k_grouped(const GroupTask* __restrict__ tasks,
+__global__ __launch_bounds__(256, 2) void k_grouped(const GroupTask* __restrict__ tasks,
                                                  const int64_t* __restrict__ starts, int ntasks,
                                                  int64_t total, unsigned long long* counters) {
     constexpr int kValuSmem = gemm_smem_floats<C::bm, C::bn, C::bk>();
@@ -204,9 +204,11 @@ const GroupedConfig kGConfigs[] = {
     LOF_GCFG(64, 64, 4, 4, 16, 8, 1),
     LOF_GCFG(32, 64, 2, 4, 32, 4, 1),
     LOF_GCFG(32, 64, 2, 4, 32, 8, 1),
+    LOF_GCFG(64, 64, 4, 4, 16, 4, 2),
+    LOF_GCFG(64, 64, 4, 4, 16, 8, 2),
 };
 constexpr int kNumGConfigs = sizeof(kGConfigs) / sizeof(kGConfigs[0]);
-int g_default_gconfig = 0;
+int g_default_gconfig = 4;   // 64 x 64 VALU, 128 x 64 MFMA, 4 rounding sub-tiles
 std::mutex g_gmu;
 
 }  // namespace

@@ -75,7 +75,7 @@ def _check(jobs, refs, what):
         assert_bits(j.out.cpu().numpy().ravel(), np.asarray(r).ravel(), f"{what} job {k}")
 
 
-@pytest.mark.parametrize("config", [0, 1, 2, 3])
+@pytest.mark.parametrize("config", range(6))
 def test_grouped_matches_oracle(dev, config):
     import lofloat_amd as lof
     jobs, refs = _make_jobs(lof, dev, seed=config)
