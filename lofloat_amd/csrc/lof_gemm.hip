@@ -3,6 +3,7 @@
 // lof_gemm_autotune).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -96,14 +97,14 @@ TuneKey tune_key(const lof_gemm_desc& d) {
 }
 
 // Untuned choice, from measurements at M = N = K = 4096 and on conv-shaped GEMMs
-// (M = 64..512, e4m3 / 8e7m inputs): wave tiles for the chunks they take (64x64 for c <= 8,
-// where the leaves dominate, chunk 1 included; 128x64 for c >= 16 when that still gives
+// (M = 64..512, e4m3 / 8e7m inputs): wave tiles for the chunks they take (64x64 for c <= 4,
+// where the leaves dominate, chunk 1 included; 128x64 for c >= 8 when that still gives
 // >= 384 tiles, 1.5 per CU), the 32x64 VALU tile otherwise; the 32x32 VALU tile when there
 // are too few tiles to fill the chip.
 int default_config(const lof_gemm_desc& d) {
     if (chunk_fits_wave_tile(d.accum.chunk)) {
         const int64_t tiles_128x64 = (int64_t)((d.M + 127) / 128) * ((d.N + 63) / 64);
-        return (d.accum.chunk >= 16 && tiles_128x64 >= 384) ? kMfma128x64 : kMfma64;
+        return (d.accum.chunk >= 8 && tiles_128x64 >= 384) ? kMfma128x64 : kMfma64;
     }
     if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 1024) return 3;
     return 4;
@@ -148,6 +149,24 @@ static void set_fast_domain(GemmArgs* g, int elo, int ehi, int m) {
     g->elo_b = elo + 127;
     g->elo_d = elo + 1023;
     g->subn = lo_e < elo ? 1 : 0;
+    g->sum_ehi = ehi;
+}
+
+// Bounds for the MFMA tile's exact-sum leaves.  q: every value of the accumulator format is
+// a multiple of 2^q (its smallest subnormal; -149 for a mantissa-only accumulator).
+// sum_g = (1 + 2^-(m+1))^n for the n = 32 / chunk leaves of one K slice, rounded up to a
+// float: one round-to-nearest step grows a magnitude by at most the factor 1 + 2^-(m+1).
+static void set_exact_sum_bounds(GemmArgs* g, int m, int q) {
+    if (q < -149) q = -149;
+    g->sum_q = q;
+    g->sum_lreq = q > -126 ? q : -126;
+    g->sum_m = m;
+    const int n = g->chunk >= 32 ? 1 : 32 / g->chunk;
+    const double gr = std::pow(1.0 + std::ldexp(1.0, -(m + 1)), n);
+    float gf = (float)gr;
+    if ((double)gf < gr) gf = std::nextafter(gf, 1e30f);
+    g->sum_g = gf;
+    g->sum_n = (float)n;
 }
 
 lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
@@ -213,6 +232,7 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
     g->vec = (((uintptr_t)d->A | (uintptr_t)d->B) & 15u) == 0 && (d->lda & 3) == 0 &&
              (d->ldb & 3) == 0;
     set_fast_path(g, ac.spec.mode == RNE && m <= 19, m);
+    set_exact_sum_bounds(g, m, ac.kind == LOF_ACCUM_FORMAT ? 1 - ac.fmt.bias - m : -149);
     return LOF_OK;
 }
 

@@ -101,6 +101,13 @@ struct GemmArgs {
     int32_t elo_b;       // smallest normal exponent of the format, FP32-biased
     int32_t elo_d;       // the same, FP64-biased
     int32_t subn;        // the float fast path covers part of the format's subnormal range
+    // exact-sum leaves of the MFMA tile (used when range_f != 0)
+    float sum_g;         // (1 + 2^-(m+1))^n rounded up, n = leaves per K slice (32 / chunk)
+    float sum_n;         // n
+    int32_t sum_ehi;     // |s| must stay below 2^sum_ehi (the format's top binade excluded)
+    int32_t sum_q;       // every format value is a multiple of 2^sum_q
+    int32_t sum_lreq;    // products must be multiples of 2^sum_lreq = max(sum_q, -126)
+    int32_t sum_m;       // the format's mantissa bits
     int32_t vec;         // A, B 16-byte aligned with lda, ldb multiples of 4 (float4 loads)
     int32_t mfma_mode;   // MFMA tile: 0 exactness test, 1 every chunk on the VALU chain,
                          // 2 every chunk on MFMA (testing only: skips the test)

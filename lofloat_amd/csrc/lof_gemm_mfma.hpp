@@ -100,6 +100,19 @@ __device__ __forceinline__ bool leaf_f_zero(float p, float acc, const GemmArgs& 
     return true;
 }
 
+// Exact-sum leaf on the negated accumulator (the tile keeps nacc = -acc): t = nacc - p =
+// -(acc + p), computed exactly in FP32 (the caller has proved it), so RTO24 is the identity
+// and the leaf is round(acc + p) = -round(t): round-to-nearest-even at bit rf of t's
+// pattern, which is right in the format's normal range and leaves a value in its subnormal
+// range unchanged (the caller has also proved such sums lie on the format's grid).  The
+// rounding keeps the sign, so the result is the new nacc.  An exact zero t gets IEEE's
+// round-to-nearest sign, which for t = -(acc + p) is roundTowardNegative's sign of acc + p:
+// the reference's zero rule ((+0) + (+0) = +0, any other exact zero is -0).
+__device__ __forceinline__ float leaf_exact(float t, const GemmArgs& g) {
+    const uint32_t b = __float_as_uint(t);
+    return __uint_as_float((b + g.half_m1f + ((b >> g.rf) & 1u)) & g.keep_maskf);
+}
+
 // Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
 // results are committed where valid, outputs needing the general path are recorded in a bit
 // mask, and one wave-uniform branch runs the general path for them (acc still holds their
@@ -132,7 +145,7 @@ struct MfmaTile {
     static constexpr int LDA_S = BM + 4;   // As[k][m], rows padded to keep float4 alignment
     static constexpr int LDB_S = BN + 4;   // Bs[k][n]
     static constexpr int BUF_FLOATS = BK * LDA_S + BK * LDB_S;
-    static constexpr int STAT_WORDS = 4 * 6;   // [wave][amaxA, minA, orA, amaxB, minB, orB]
+    static constexpr int STAT_WORDS = 4 * 8;   // [wave][amaxA, minA, orA, amaxB, minB, orB, keyA, keyB]
     static constexpr int SMEM_FLOATS = 2 * (BUF_FLOATS + STAT_WORDS);
     static constexpr int NA4 = BM * BK / 1024;   // float4 of the A slice per thread
     static constexpr int NB4 = BN * BK / 1024;
@@ -143,14 +156,24 @@ struct MfmaTile {
 //   mnk  = min((|v| << 1) - 1)     the smallest nonzero magnitude (zeros wrap to ~0)
 //   orb  = OR of the patterns      its low 23 bits give the smallest trailing-zero count
 // so every nonzero value's lowest set bit is at exponent >= e_min - 150 + ctz(orb | 2^23).
+// With KEY, also
+//   key  = min(E + min(ctz(v), 24)), E = ((|v| << 1) - 1) >> 24
+// For an FP32-normal v with a nonzero fraction, E is its biased exponent and ctz(v) the
+// trailing-zero count of the fraction, so key - 150 is the exponent of its lowest set bit;
+// for a power of two E is one less and min(ctz, 24) is 24 (even exponent) or 23 (odd), so
+// key - 150 is that exponent or one below it.  Either way key - 150 bounds every normal
+// value's lowest set bit from below; zeros give 279 and drop out of the minimum.
+template <bool KEY>
 struct SliceStat {
-    uint32_t amax = 0u, mnk = 0xFFFFFFFFu, orb = 0u;
+    uint32_t amax = 0u, mnk = 0xFFFFFFFFu, orb = 0u, key = 0xFFFFFFFFu;
     __device__ __forceinline__ void add(float v) {
         const uint32_t b = __float_as_uint(v);
         const uint32_t sh = b << 1;
+        const uint32_t nz = sh - 1u;
         amax = max(amax, sh);
-        mnk = min(mnk, sh - 1u);
+        mnk = min(mnk, nz);
         orb |= b;
+        if constexpr (KEY) key = min(key, (nz >> 24) + min((uint32_t)__builtin_ctz(b | 0x80000000u), 24u));
     }
     __device__ __forceinline__ void add(const float4& v) { add(v.x); add(v.y); add(v.z); add(v.w); }
 };
@@ -178,6 +201,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     constexpr int LOG2C = CHUNK == 1 ? 0 : CHUNK == 2 ? 1 : CHUNK == 4 ? 2 : CHUNK == 8 ? 3
                         : CHUNK == 16 ? 4 : 5;
     static_assert(CHUNK >= 1 && CHUNK <= BK && (BK % CHUNK) == 0, "chunk must divide 32");
+    // per-value lowest-bit statistics for the short chunks, where the exact-sum leaves
+    // carry most of the work (measured: they cost more than they win from chunk 8 up)
+    constexpr bool kKey = CHUNK <= 4;
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -186,14 +212,29 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     const int am0 = wm * 32 * WMB, bn0 = wn * 32 * WNB;   // this wave's sub-tile
     const bool interior_mn = m0 + BM <= g.M && n0 + BN <= g.N;
 
-    float acc[WMB][WNB][16];
+    // The lane's accumulators, held as acc (negd false) for the general leaves or as -acc
+    // (negd true) for the exact-sum leaves, which work on the negation directly; a lane
+    // flips them only when it changes from one kind of slice to the other.
+    float nacc[WMB][WNB][16];
+    bool negd = false;
     f32x16 part[WMB][WNB];
 #pragma unroll
     for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
         for (int bj = 0; bj < WNB; ++bj)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[bi][bj][r] = 0.0f;
+            for (int r = 0; r < 16; ++r) nacc[bi][bj][r] = 0.0f;
+    auto set_form = [&](bool neg) {
+        if (neg != negd) {
+#pragma unroll
+            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) nacc[bi][bj][r] = -nacc[bi][bj][r];
+            negd = neg;
+        }
+    };
 
     // Slice element mapping: float4 f = tid + it * 256 of the A slice holds
     //   ta = 0 (A row-major, k contiguous): row m = f / 8, k = 4 * (f % 8) .. + 3
@@ -321,7 +362,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         float* As = smem + buf * (T::BUF_FLOATS + T::STAT_WORDS);
         float* Bs = As + BK * LDA_S;
         uint32_t* stat = (uint32_t*)(Bs + BK * LDB_S);
-        SliceStat sa, sb;
+        SliceStat<kKey> sa, sb;
 #pragma unroll
         for (int it = 0; it < NA4; ++it) {
             int m, k;
@@ -356,10 +397,15 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         sb.mnk = wave_min_u32(sb.mnk);
         sa.orb = wave_or_u32(sa.orb);
         sb.orb = wave_or_u32(sb.orb);
+        if constexpr (kKey) {
+            sa.key = wave_min_u32(sa.key);
+            sb.key = wave_min_u32(sb.key);
+        }
         if (lane == 0) {
-            uint32_t* st = stat + wave * 6;
+            uint32_t* st = stat + wave * 8;
             st[0] = sa.amax; st[1] = sa.mnk; st[2] = sa.orb;
             st[3] = sb.amax; st[4] = sb.mnk; st[5] = sb.orb;
+            st[6] = sa.key; st[7] = sb.key;
         }
     };
 
@@ -376,21 +422,34 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         const float* Bs = As + BK * LDA_S;
         const uint32_t* stat = (const uint32_t*)(Bs + BK * LDB_S);
         bool exact;
+        // for the exact-sum test: every product (chunk 1) or chunk sum of the slice is a
+        // multiple of 2^sL and below 2^sTop in magnitude; sfin: no inf or NaN in the slice
+        int sL = 0, sTop = 0;
+        bool sfin = false;
         {
             uint32_t amA = 0u, mnA = 0xFFFFFFFFu, orA = 0u, amB = 0u, mnB = 0xFFFFFFFFu, orB = 0u;
+            uint32_t kyA = 0xFFFFFFFFu, kyB = 0xFFFFFFFFu;
 #pragma unroll
             for (int w = 0; w < 4; ++w) {
-                amA = max(amA, stat[w * 6 + 0]); mnA = min(mnA, stat[w * 6 + 1]); orA |= stat[w * 6 + 2];
-                amB = max(amB, stat[w * 6 + 3]); mnB = min(mnB, stat[w * 6 + 4]); orB |= stat[w * 6 + 5];
+                amA = max(amA, stat[w * 8 + 0]); mnA = min(mnA, stat[w * 8 + 1]); orA |= stat[w * 8 + 2];
+                amB = max(amB, stat[w * 8 + 3]); mnB = min(mnB, stat[w * 8 + 4]); orB |= stat[w * 8 + 5];
+                if constexpr (kKey) { kyA = min(kyA, stat[w * 8 + 6]); kyB = min(kyB, stat[w * 8 + 7]); }
             }
             if (amA == 0u || amB == 0u) {
                 exact = true;                              // every product is zero
+                sfin = amA < 0xFF000000u && amB < 0xFF000000u;
+                sL = 1000;
+                sTop = -1000;
             } else if (amA >= 0xFF000000u || amB >= 0xFF000000u) {
                 exact = false;                             // inf or NaN in the slice
             } else {
                 int lsbA, lsbB, eA, eB;
                 const bool normA = slice_bounds(amA, mnA, orA, lsbA, eA);
                 const bool normB = slice_bounds(amB, mnB, orB, lsbB, eB);
+                if constexpr (kKey) {   // the per-value bound, tighter than slice_bounds'
+                    lsbA = max(lsbA, (int)kyA - 150);
+                    lsbB = max(lsbB, (int)kyB - 150);
+                }
                 const int L = lsbA + lsbB;
                 if constexpr (CHUNK == 1) {
                     // every product a*b exact in FP32: significand widths sum to <= 24, and
@@ -398,18 +457,93 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     const int wA = 24 - __builtin_ctz((orA & 0x7FFFFFu) | 0x800000u);
                     const int wB = 24 - __builtin_ctz((orB & 0x7FFFFFu) | 0x800000u);
                     exact = normA && normB && wA + wB <= 24 && eA + eB + 2 <= 127 && L >= -149;
+                    sTop = eA + eB + 2;
                 } else {
                     const int top = LOG2C + eA + eB + 2;
                     exact = normA && normB && L >= -126 && top <= L + 24 && top <= 127;
+                    sTop = top;
                 }
+                sL = L;
+                sfin = true;
             }
-            if (g.mfma_mode != 0) exact = g.mfma_mode == 2;
+            if (g.mfma_mode != 0) {
+                exact = g.mfma_mode == 2;
+                sfin = false;
+            }
         }
         const bool has_next = k0 + BK < g.K;
         if (has_next) load_slice(k0 + BK);   // in flight while this slice computes
 
+        // Exact-sum test, per lane: is every sum s = acc + p of this slice exact in FP32?
+        // With |acc| < 2^ea and |p| < 2^sTop at the slice start, a leaf grows |acc| to at
+        // most (|acc| + 2^sTop)(1 + 2^-(m+1)), so over the slice's n leaves every s is below
+        // g (2^ea + n 2^sTop) + 2^sTop < 2^S, g = (1 + 2^-(m+1))^n (rounded up on the host,
+        // and the float evaluation given a 2^-20 margin).  Every acc is a format value, a multiple of
+        // 2^max(E(acc) - m, q), and at the start a multiple of 2^Lf, Lf = min(sL, lowest
+        // such bound over the lane's nonzero acc); an exact sum keeps both properties, so by
+        // induction every s is exact when
+        //   S <= sL + 24       (acc a multiple of 2^sL: s is one too, with < 24 bits)
+        //   sTop <= Lf + 23    (a finer acc: s < 2^(max(E(acc) + 2, sTop + 1)) has < 24 bits
+        //                       above acc's lowest bit, as m <= 19)
+        // and the leaf formula applies when S <= sum_ehi (no top binade), Lf >= -126 (s is
+        // zero or FP32-normal) and sL >= q (sums in the format's subnormal range are on its
+        // grid).
+        bool sum_ok = false;
+        if (exact && sfin && g.range_f != 0u && sL >= g.sum_lreq) {
+            uint32_t amax = 0u, mnk = 0xFFFFFFFFu;
+#pragma unroll
+            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const uint32_t b = __float_as_uint(nacc[bi][bj][r]) & 0x7FFFFFFFu;
+                        amax = max(amax, b);
+                        mnk = min(mnk, b - 1u);            // zeros wrap to ~0
+                    }
+            const int ea = amax != 0u ? (int)(amax >> 23) - 126 : -1000;   // inf / NaN: 129
+            const int lsb_acc = mnk != 0xFFFFFFFFu
+                                    ? max(g.sum_q, (int)((mnk + 1u) >> 23) - 127 - g.sum_m)
+                                    : 1000;
+            const int Lf = min(sL, lsb_acc);
+            const float pa = ea > -1000 ? __builtin_ldexpf(1.0f, max(ea, -126)) : 0.0f;
+            const float pp = sTop > -1000 ? __builtin_ldexpf(1.0f, max(sTop, -126)) : 0.0f;
+            const float bnd = __builtin_fmaf(g.sum_g, __builtin_fmaf(g.sum_n, pp, pa), pp) *
+                              1.00000095367431640625f;   // 1 + 2^-20
+            const int S = (int)(__float_as_uint(bnd) >> 23) - 126;
+            sum_ok = S <= min(sL + 24, g.sum_ehi) && sTop <= Lf + 23 && Lf >= -126;
+        }
+
         const int kmax = min(BK, g.K - k0);
         if constexpr (CHUNK == 1) {
+          set_form(sum_ok);
+          if (sum_ok) {
+            // nacc <- -round(a * b + acc): fma(-a, b, nacc) is -(a * b + acc) exactly
+            for (int kk = 0; kk < kmax; ++kk) {
+                float4 a4[WMB][4];
+                float b[WNB];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq)
+                        a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj) b[bj] = Bs[kk * LDB_S + bn0 + bj * 32 + l32];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                        for (int rq = 0; rq < 4; ++rq) {
+                            float* na = &nacc[bi][bj][rq * 4];
+                            na[0] = leaf_exact(__builtin_fmaf(-a4[bi][rq].x, b[bj], na[0]), g);
+                            na[1] = leaf_exact(__builtin_fmaf(-a4[bi][rq].y, b[bj], na[1]), g);
+                            na[2] = leaf_exact(__builtin_fmaf(-a4[bi][rq].z, b[bj], na[2]), g);
+                            na[3] = leaf_exact(__builtin_fmaf(-a4[bi][rq].w, b[bj], na[3]), g);
+                        }
+            }
+          } else {
+            float (&acc)[WMB][WNB][16] = nacc;
             // acc <- round(RTO_fma(a, b, acc)) for every k: float products where the slice's
             // products are exact in FP32, double products otherwise
             for (int kk = 0; kk < kmax; ++kk) {
@@ -454,7 +588,10 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         }
                 }
             }
+          }
         } else {
+        set_form(sum_ok);
+        float (&acc)[WMB][WNB][16] = nacc;
 #pragma unroll
         for (int q = 0; q < BK / CHUNK; ++q) {
             const int kb = q * CHUNK;
@@ -525,10 +662,20 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                             }
                 }
             }
+            if (sum_ok) {   // nacc <- -round(p + acc) = round(nacc - p), the sum exact
 #pragma unroll
-            for (int bi = 0; bi < WMB; ++bi)
+                for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
+                    for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                        for (int r = 0; r < 16; ++r)
+                            nacc[bi][bj][r] = leaf_exact(nacc[bi][bj][r] - part[bi][bj][r], g);
+            } else {
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
+            }
         }
         }
         if (has_next) store_slice(buf ^ 1);
@@ -551,7 +698,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             for (int r = 0; r < 16; ++r) {
                 const int gm = m0 + am0 + bi * 32 + (r >> 2) * 8 + lhi * 4 + (r & 3);
                 if (gm >= g.M || gn >= g.N) continue;
-                const float a = acc[bi][bj][r];
+                const float a = negd ? -nacc[bi][bj][r] : nacc[bi][bj][r];
                 float d;
                 if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * g.C[(int64_t)gm * row_stride_c + col]);
                 else d = g.alpha * a;

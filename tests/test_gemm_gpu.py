@@ -261,6 +261,28 @@ def test_zero_sums_every_config(dev, chunk):
                             f"zero sums chunk {chunk} config {cfg} general leaf")
 
 
+@pytest.mark.parametrize("chunk", [1, 2, 8, 32])
+def test_exact_sum_transitions(dev, chunk):
+    """Slices and lanes moving between the exact-sum leaves and the general ones: rows 0-7
+    start with 2^-12-scaled values (their accumulators stay tiny, with low bits a
+    mantissa-only accumulator cannot add exactly to the products that follow) before normal
+    e4m3 data, and k 160..191 of rows 16-23 are 2^-14-scaled (those tiles' slice fails the
+    test for every lane), so lanes change kind of slice both ways mid-product."""
+    import lofloat_amd as lof
+    M, N, K = 96, 80, 320
+    A, B = _mats(M, N, K, seed=chunk, kind="e4m3_small")
+    A[:8, :64] = oracle.virtual_round(A[:8, :64] * np.float32(2.0 ** -12), E4M3).reshape(8, 64)
+    A[16:24, 160:192] = oracle.virtual_round(A[16:24, 160:192] * np.float32(2.0 ** -14),
+                                             E4M3).reshape(8, 32)
+    for acc, oacc in (_accums(lof, 10, chunk), _accums(lof, fmt=FP16_SAT, chunk=chunk),
+                      _accums(lof, fmt=FP16_EXT, chunk=chunk)):
+        ref = _ref(A, B, oacc)
+        for cfg in range(lof.gemm_num_configs()):
+            if lof.gemm_config_valid(cfg, chunk):
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref,
+                            f"transitions chunk {chunk} config {cfg}")
+
+
 def test_alpha_and_reference_entry(dev):
     import lofloat_amd as lof
     A, B = _mats(64, 48, 96, seed=2)
