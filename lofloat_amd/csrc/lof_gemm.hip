@@ -233,6 +233,12 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
              (d->ldb & 3) == 0;
     set_fast_path(g, ac.spec.mode == RNE && m <= 19, m);
     set_exact_sum_bounds(g, m, ac.kind == LOF_ACCUM_FORMAT ? 1 - ac.fmt.bias - m : -149);
+    // a binary16-shaped format (10 mantissa bits, bias 15) rounds like the hardware's
+    // binary16 conversion wherever |s| < 2^15, which the exact-sum test then requires
+    if (ac.kind == LOF_ACCUM_FORMAT && m == 10 && ac.fmt.bias == 15) {
+        g->sum_f16 = 1;
+        if (g->sum_ehi > 15) g->sum_ehi = 15;
+    }
     return LOF_OK;
 }
 

@@ -32,6 +32,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "lof_gemm_kernels.hpp"
 
 namespace lofamd {
@@ -108,9 +110,17 @@ __device__ __forceinline__ bool leaf_f_zero(float p, float acc, const GemmArgs& 
 // rounding keeps the sign, so the result is the new nacc.  An exact zero t gets IEEE's
 // round-to-nearest sign, which for t = -(acc + p) is roundTowardNegative's sign of acc + p:
 // the reference's zero rule ((+0) + (+0) = +0, any other exact zero is -0).
+// F16: the accumulator's grid is IEEE binary16's below 2^15 (m = 10, bias 15) and the
+// caller has bounded |t| below 2^15, so the hardware conversion pair (round-to-nearest-even,
+// binary16 subnormals kept) computes the same rounding in two instructions.
+template <bool F16>
 __device__ __forceinline__ float leaf_exact(float t, const GemmArgs& g) {
-    const uint32_t b = __float_as_uint(t);
-    return __uint_as_float((b + g.half_m1f + ((b >> g.rf) & 1u)) & g.keep_maskf);
+    if constexpr (F16) {
+        return (float)(_Float16)t;
+    } else {
+        const uint32_t b = __float_as_uint(t);
+        return __uint_as_float((b + g.half_m1f + ((b >> g.rf) & 1u)) & g.keep_maskf);
+    }
 }
 
 // Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
@@ -517,8 +527,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         const int kmax = min(BK, g.K - k0);
         if constexpr (CHUNK == 1) {
           set_form(sum_ok);
-          if (sum_ok) {
-            // nacc <- -round(a * b + acc): fma(-a, b, nacc) is -(a * b + acc) exactly
+          // nacc <- -round(a * b + acc): fma(-a, b, nacc) is -(a * b + acc) exactly
+          auto exact_c1 = [&](auto f16) {
+            constexpr bool F16 = decltype(f16)::value;
             for (int kk = 0; kk < kmax; ++kk) {
                 float4 a4[WMB][4];
                 float b[WNB];
@@ -536,12 +547,16 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
 #pragma unroll
                         for (int rq = 0; rq < 4; ++rq) {
                             float* na = &nacc[bi][bj][rq * 4];
-                            na[0] = leaf_exact(__builtin_fmaf(-a4[bi][rq].x, b[bj], na[0]), g);
-                            na[1] = leaf_exact(__builtin_fmaf(-a4[bi][rq].y, b[bj], na[1]), g);
-                            na[2] = leaf_exact(__builtin_fmaf(-a4[bi][rq].z, b[bj], na[2]), g);
-                            na[3] = leaf_exact(__builtin_fmaf(-a4[bi][rq].w, b[bj], na[3]), g);
+                            na[0] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].x, b[bj], na[0]), g);
+                            na[1] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].y, b[bj], na[1]), g);
+                            na[2] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].z, b[bj], na[2]), g);
+                            na[3] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].w, b[bj], na[3]), g);
                         }
             }
+          };
+          if (sum_ok) {
+            if (g.sum_f16) exact_c1(std::true_type{});
+            else exact_c1(std::false_type{});
           } else {
             float (&acc)[WMB][WNB][16] = nacc;
             // acc <- round(RTO_fma(a, b, acc)) for every k: float products where the slice's
@@ -663,13 +678,23 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 }
             }
             if (sum_ok) {   // nacc <- -round(p + acc) = round(nacc - p), the sum exact
+                if (g.sum_f16) {
 #pragma unroll
-                for (int bi = 0; bi < WMB; ++bi)
+                    for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < WNB; ++bj)
+                        for (int bj = 0; bj < WNB; ++bj)
 #pragma unroll
-                        for (int r = 0; r < 16; ++r)
-                            nacc[bi][bj][r] = leaf_exact(nacc[bi][bj][r] - part[bi][bj][r], g);
+                            for (int r = 0; r < 16; ++r)
+                                nacc[bi][bj][r] = leaf_exact<true>(nacc[bi][bj][r] - part[bi][bj][r], g);
+                } else {
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r)
+                                nacc[bi][bj][r] = leaf_exact<false>(nacc[bi][bj][r] - part[bi][bj][r], g);
+                }
             } else {
 #pragma unroll
                 for (int bi = 0; bi < WMB; ++bi)
