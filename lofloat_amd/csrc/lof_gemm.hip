@@ -57,9 +57,10 @@ const GemmConfig kConfigs[] = {
     LOF_GEMM_MFMA_CFG(2, 2),
     LOF_GEMM_MFMA_CFG(2, 1),
     LOF_GEMM_MFMA_CFG(1, 1),
+    LOF_GEMM_MFMA_CFG(1, 2),
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
-constexpr int kMfma128x64 = 6, kMfma64 = 7;
+constexpr int kMfma64 = 7, kMfma64x128 = 8;
 
 // chunks whose sums can run on MFMA
 bool chunk_fits_mfma(int chunk) { return chunk >= 2 && chunk <= 32 && (32 % chunk) == 0; }
@@ -98,13 +99,13 @@ TuneKey tune_key(const lof_gemm_desc& d) {
 
 // Untuned choice, from measurements at M = N = K = 4096 and on conv-shaped GEMMs
 // (M = 64..512, e4m3 / 8e7m inputs): wave tiles for the chunks they take (64x64 for c <= 4,
-// where the leaves dominate, chunk 1 included; 128x64 for c >= 8 when that still gives
-// >= 384 tiles, 1.5 per CU), the 32x64 VALU tile otherwise; the 32x32 VALU tile when there
-// are too few tiles to fill the chip.
+// where the leaves dominate, chunk 1 included; 64x128 for c >= 8 when that still gives
+// >= 384 tiles, 1.5 per CU: each lane's A values serve two B columns), the 32x64 VALU tile
+// otherwise; the 32x32 VALU tile when there are too few tiles to fill the chip.
 int default_config(const lof_gemm_desc& d) {
     if (chunk_fits_wave_tile(d.accum.chunk)) {
-        const int64_t tiles_128x64 = (int64_t)((d.M + 127) / 128) * ((d.N + 63) / 64);
-        return (d.accum.chunk >= 8 && tiles_128x64 >= 384) ? kMfma128x64 : kMfma64;
+        const int64_t tiles_64x128 = (int64_t)((d.M + 63) / 64) * ((d.N + 127) / 128);
+        return (d.accum.chunk >= 8 && tiles_64x128 >= 384) ? kMfma64x128 : kMfma64;
     }
     if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 1024) return 3;
     return 4;
@@ -283,7 +284,9 @@ LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_s
     if (s != LOF_OK) return s;
     // testing bits of config: 0x100 forces the general leaf path; 0x200 runs every chunk of
     // an MFMA config on the VALU chain; 0x400 runs every chunk on MFMA without the exactness
-    // test (results then differ where a chunk is inexact)
+    // test (results then differ where a chunk is inexact); 0x800 turns the MFMA tile's
+    // exact-sum leaves off
+    if (config >= 0 && (config & 0x800)) g.sum_off = 1;
     if (config >= 0 && (config & 0x100)) set_fast_path(&g, false, 0);
     if (config >= 0 && (config & 0x200)) g.mfma_mode = 1;
     else if (config >= 0 && (config & 0x400)) g.mfma_mode = 2;

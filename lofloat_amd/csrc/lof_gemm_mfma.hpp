@@ -499,7 +499,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // zero or FP32-normal) and sL >= q (sums in the format's subnormal range are on its
         // grid).
         bool sum_ok = false;
-        if (exact && sfin && g.range_f != 0u && sL >= g.sum_lreq) {
+        if (exact && sfin && g.range_f != 0u && sL >= g.sum_lreq && !g.sum_off) {
             uint32_t amax = 0u, mnk = 0xFFFFFFFFu;
 #pragma unroll
             for (int bi = 0; bi < WMB; ++bi)

@@ -101,7 +101,7 @@ def test_shapes_all_configs(dev, shape):
                         f"{shape} {kind} chunk {chunk} config {cfg}")
 
 
-MFMA_CONFIGS = (5, 6, 7)
+MFMA_CONFIGS = (5, 6, 7, 8)
 
 
 @pytest.mark.parametrize("kind", ["e4m3_small", "mixed", "normal", "special", "e4m3", "tiny"])
@@ -281,6 +281,9 @@ def test_exact_sum_transitions(dev, chunk):
             if lof.gemm_config_valid(cfg, chunk):
                 assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref,
                             f"transitions chunk {chunk} config {cfg}")
+        for cfg in MFMA_CONFIGS:   # 0x800: the same tiles with the exact-sum leaves off
+            assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x800), ref,
+                        f"transitions chunk {chunk} config {cfg} without exact-sum leaves")
 
 
 def test_alpha_and_reference_entry(dev):
