@@ -23,8 +23,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                                         smem);
 }
 
+// The 64x64 chunk-1 tile is held to 128 registers, 4 waves per SIMD (a few spills): its
+// exact-sum leaves are latency-bound, and 4096 tiles then fill 1024 slots exactly.
 template <int WMB, int WNB, int CHUNK, bool SUBN>
-__global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs g) {
+__global__ __launch_bounds__(256, (WMB == 1 && WNB == 1 && CHUNK == 1) ? 4 : 1)
+void k_gemm_mfma(GemmArgs g) {
     __shared__ __align__(16) float smem[MfmaTile<WMB, WNB>::SMEM_FLOATS];
     gemm_tile_mfma<WMB, WNB, CHUNK, SUBN>(g, (int)blockIdx.y, (int)blockIdx.x, (int)threadIdx.x,
                                           smem);
