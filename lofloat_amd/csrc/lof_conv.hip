@@ -20,9 +20,9 @@ namespace lofamd {
 
 namespace {
 
-// the 64x64 chunk-1 tile at 4 waves per SIMD, as the GEMM's
+// the 64x64 tile at 4 waves per SIMD, as the GEMM's
 template <int WMB, int WNB, int CHUNK, bool SUBN>
-__global__ __launch_bounds__(256, (WMB == 1 && CHUNK == 1) ? 4 : 1)
+__global__ __launch_bounds__(256, WMB == 1 ? 4 : 1)
 void k_conv_mfma(GemmArgs g) {
     __shared__ __align__(16) float smem[MfmaTile<WMB, WNB>::SMEM_FLOATS];
     gemm_tile_mfma<WMB, WNB, CHUNK, SUBN, true>(g, (int)blockIdx.y, (int)blockIdx.x,

@@ -23,10 +23,11 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                                         smem);
 }
 
-// The 64x64 chunk-1 tile is held to 128 registers, 4 waves per SIMD (a few spills): its
-// exact-sum leaves are latency-bound, and 4096 tiles then fill 1024 slots exactly.
+// Occupancy: the 64x64 tile is held to 128 registers (4 waves per SIMD; chunk 1 spills a
+// few), the 64x128 tile to 168 (3 waves): both are latency-bound, and a 4096^2 output's
+// 4096 64x64 tiles then fill 1024 slots in exactly four rounds.
 template <int WMB, int WNB, int CHUNK, bool SUBN>
-__global__ __launch_bounds__(256, (WMB == 1 && WNB == 1 && CHUNK == 1) ? 4 : 1)
+__global__ __launch_bounds__(256, (WMB == 1 && WNB == 1) ? 4 : (WMB == 1 && WNB == 2) ? 3 : 1)
 void k_gemm_mfma(GemmArgs g) {
     __shared__ __align__(16) float smem[MfmaTile<WMB, WNB>::SMEM_FLOATS];
     gemm_tile_mfma<WMB, WNB, CHUNK, SUBN>(g, (int)blockIdx.y, (int)blockIdx.x, (int)threadIdx.x,
