@@ -51,10 +51,11 @@ constexpr int kRVPT = 2;  // float4 slots per lane per sub-tile
 
 static_assert(256 * kRVPT * 4 == kMxTileElems, "rounding and MX sub-tiles are 2048 floats");
 
-template <int BM, int BN, int TM, int TN, int BK, int SUB, int WMB>
+template <int BM, int BN, int TM, int TN, int BK, int SUB, int WMB, int WNB>
 struct GroupedCfg {
     static constexpr int bm = BM, bn = BN, tm = TM, tn = TN, bk = BK, sub = SUB;
-    static constexpr int mfma_wmb = WMB;   // MFMA GEMM tile: (64 * WMB) x 64
+    static constexpr int mfma_wmb = WMB;   // MFMA GEMM tile: (64 * WMB) x (64 * WNB)
+    static constexpr int mfma_wnb = WNB;
     // floats per rounding / MX tile: SUB sub-tiles of 2048 floats
     static constexpr int64_t round_tile_elems = (int64_t)SUB * kMxTileElems;
 };
@@ -111,20 +112,20 @@ __device__ __noinline__ void grouped_mx(const MxArgs& a, int mode, int rsrc, int
 // GEMM tiles whose chunk sums run on MFMA (chunks 8, 16, 32): the 64 x 64 tile per
 // (chunk, subnormal fast range) variant, inlined (as non-inlined calls they raise the kernel's register count
 // enough to halve its occupancy).
-template <int WMB, int CHUNK, bool SUBN>
+template <int WMB, int WNB, int CHUNK, bool SUBN>
 __device__ __forceinline__ void g_mfma_tile(const GemmArgs& g, int tm, int tn, int tid,
                                          float* smem) {
-    gemm_tile_mfma<WMB, 1, CHUNK, SUBN>(g, tm, tn, tid, smem);
+    gemm_tile_mfma<WMB, WNB, CHUNK, SUBN>(g, tm, tn, tid, smem);
 }
 
-template <int WMB>
+template <int WMB, int WNB>
 __device__ __forceinline__ void grouped_mfma(const GemmArgs& g, int tm, int tn, int tid,
                                           float* smem) {
-#define LOF_GM(C)                                                        \
-    if (g.chunk == C) {                                                  \
-        if (g.subn) g_mfma_tile<WMB, C, true>(g, tm, tn, tid, smem);     \
-        else g_mfma_tile<WMB, C, false>(g, tm, tn, tid, smem);           \
-        return;                                                          \
+#define LOF_GM(C)                                                             \
+    if (g.chunk == C) {                                                       \
+        if (g.subn) g_mfma_tile<WMB, WNB, C, true>(g, tm, tn, tid, smem);     \
+        else g_mfma_tile<WMB, WNB, C, false>(g, tm, tn, tid, smem);           \
+        return;                                                               \
     }
     LOF_GM(8) LOF_GM(16) LOF_GM(32)
 #undef LOF_GM
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(256, 2) void k_grouped(const GroupTask* __restrict_
                                                  const int64_t* __restrict__ starts, int ntasks,
                                                  int64_t total, unsigned long long* counters) {
     constexpr int kValuSmem = gemm_smem_floats<C::bm, C::bn, C::bk>();
-    constexpr int kMfmaSmem = MfmaTile<C::mfma_wmb, 1>::SMEM_FLOATS;
+    constexpr int kMfmaSmem = MfmaTile<C::mfma_wmb, C::mfma_wnb>::SMEM_FLOATS;
     __shared__ __align__(16) float smem[kValuSmem > kMfmaSmem ? kValuSmem : kMfmaSmem];
     __shared__ unsigned long long s_tile;
     const int tid = threadIdx.x;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256, 2) void k_grouped(const GroupTask* __restrict_
             break;
         default:
             if (T.mfma)
-                grouped_mfma<C::mfma_wmb>(T.g, (int)(local / T.tiles_n), (int)(local % T.tiles_n),
+                grouped_mfma<C::mfma_wmb, C::mfma_wnb>(T.g, (int)(local / T.tiles_n), (int)(local % T.tiles_n),
                                           tid, smem);
             else if (T.g.subn)
                 gemm_tile<C::bm, C::bn, C::tm, C::tn, C::bk, true>(
@@ -189,23 +190,25 @@ __global__ __launch_bounds__(256, 2) void k_grouped(const GroupTask* __restrict_
 struct GroupedConfig {
     const char* name;
     int bm, bn;
-    int mfma_bm;   // MFMA GEMM tile rows (64 x 64 or 128 x 64 tiles)
+    int mfma_bm, mfma_bn;   // MFMA GEMM tile (64 x 64, 128 x 64 or 64 x 128)
     int64_t round_tile_elems;
     void (*kernel)(const GroupTask*, const int64_t*, int, int64_t, unsigned long long*);
 };
 
-#define LOF_GCFG(BM, BN, TM, TN, BK, SUB, WMB)                                             \
-    {#BM "x" #BN "/" #TM "x" #TN " r" #SUB " mfma" #WMB "1", BM, BN, 64 * WMB,              \
-     GroupedCfg<BM, BN, TM, TN, BK, SUB, WMB>::round_tile_elems,                          \
-     k_grouped<GroupedCfg<BM, BN, TM, TN, BK, SUB, WMB>>}
+#define LOF_GCFG(BM, BN, TM, TN, BK, SUB, WMB, WNB)                                        \
+    {#BM "x" #BN "/" #TM "x" #TN " r" #SUB " mfma" #WMB #WNB, BM, BN, 64 * WMB, 64 * WNB,   \
+     GroupedCfg<BM, BN, TM, TN, BK, SUB, WMB, WNB>::round_tile_elems,                     \
+     k_grouped<GroupedCfg<BM, BN, TM, TN, BK, SUB, WMB, WNB>>}
 
 const GroupedConfig kGConfigs[] = {
-    LOF_GCFG(64, 64, 4, 4, 16, 4, 1),
-    LOF_GCFG(64, 64, 4, 4, 16, 8, 1),
-    LOF_GCFG(32, 64, 2, 4, 32, 4, 1),
-    LOF_GCFG(32, 64, 2, 4, 32, 8, 1),
-    LOF_GCFG(64, 64, 4, 4, 16, 4, 2),
-    LOF_GCFG(64, 64, 4, 4, 16, 8, 2),
+    LOF_GCFG(64, 64, 4, 4, 16, 4, 1, 1),
+    LOF_GCFG(64, 64, 4, 4, 16, 8, 1, 1),
+    LOF_GCFG(32, 64, 2, 4, 32, 4, 1, 1),
+    LOF_GCFG(32, 64, 2, 4, 32, 8, 1, 1),
+    LOF_GCFG(64, 64, 4, 4, 16, 4, 2, 1),
+    LOF_GCFG(64, 64, 4, 4, 16, 8, 2, 1),
+    LOF_GCFG(64, 64, 4, 4, 16, 4, 1, 2),
+    LOF_GCFG(64, 64, 4, 4, 16, 8, 1, 2),
 };
 constexpr int kNumGConfigs = sizeof(kGConfigs) / sizeof(kGConfigs[0]);
 int g_default_gconfig = 4;   // 64 x 64 VALU, 128 x 64 MFMA, 4 rounding sub-tiles
@@ -266,7 +269,7 @@ lof_status build_task(const lof_group_item& it, const GroupedConfig& cfg, GroupT
                    it.gemm.accum.chunk == 32) ? 1 : 0;
         {
             const int bm = t->mfma ? cfg.mfma_bm : cfg.bm;
-            const int bn = t->mfma ? 64 : cfg.bn;
+            const int bn = t->mfma ? cfg.mfma_bn : cfg.bn;
             t->tiles_n = (it.gemm.N + bn - 1) / bn;
             *tiles = (int64_t)t->tiles_n * ((it.gemm.M + bm - 1) / bm);
         }

@@ -159,8 +159,10 @@ class GraphHandle:
                    "graph replay")
 
     def __del__(self):
-        if getattr(self, "_ptr", None):
-            _lib.lib.lof_graph_destroy(self._ptr)
+        # at interpreter exit the library handle may already be gone: nothing to free then
+        lib = getattr(_lib, "lib", None)
+        if getattr(self, "_ptr", None) and lib is not None:
+            lib.lof_graph_destroy(self._ptr)
             self._ptr = None
 
 
@@ -206,8 +208,9 @@ class GroupedPlan:
         return GraphHandle(g, self, side)
 
     def __del__(self):
-        if getattr(self, "_ptr", None):
-            _lib.lib.lof_grouped_destroy(self._ptr)
+        lib = getattr(_lib, "lib", None)
+        if getattr(self, "_ptr", None) and lib is not None:
+            lib.lof_grouped_destroy(self._ptr)
             self._ptr = None
 
 

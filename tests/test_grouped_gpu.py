@@ -75,7 +75,7 @@ def _check(jobs, refs, what):
         assert_bits(j.out.cpu().numpy().ravel(), np.asarray(r).ravel(), f"{what} job {k}")
 
 
-@pytest.mark.parametrize("config", range(6))
+@pytest.mark.parametrize("config", range(8))
 def test_grouped_matches_oracle(dev, config):
     import lofloat_amd as lof
     jobs, refs = _make_jobs(lof, dev, seed=config)
