@@ -109,7 +109,8 @@ struct GemmArgs {
     int32_t sum_lreq;    // products must be multiples of 2^sum_lreq = max(sum_q, -126)
     int32_t sum_m;       // the format's mantissa bits
     int32_t sum_f16;     // the format's grid is binary16's below 2^15: round by conversion
-    int32_t sum_off;     // testing: exact-sum leaves off
+    int32_t sum_rto;     // RTO leaves round this accumulator exactly (mantissa-only, binary16)
+    int32_t sum_off;     // testing: bit 0 exact-sum leaves off, bit 1 RTO leaves off
     int32_t vec;         // A, B 16-byte aligned with lda, ldb multiples of 4 (float4 loads)
     int32_t mfma_mode;   // MFMA tile: 0 exactness test, 1 every chunk on the VALU chain,
                          // 2 every chunk on MFMA (testing only: skips the test)

@@ -123,6 +123,25 @@ __device__ __forceinline__ float leaf_exact(float t, const GemmArgs& g) {
     }
 }
 
+// RTO leaf on the negated accumulator, for any (finite, in-range) sum: t = RN24(nacc - p) and
+// its TwoSum residual e give x = -(acc + p) = t + e exactly, so RTO24(x) -- the reference's
+// round-to-odd FMA result -- is t when e == 0, else the magnitude truncation of x with its
+// last bit set: t | 1 when e has t's sign (|x| > |t|), (t - 1 ulp) | 1 otherwise.  Rounding
+// that as leaf_exact does is round(RTO24(acc + p)) negated: exact wherever leaf_exact's
+// rounding is the accumulator's for every float, i.e. for a mantissa-only accumulator
+// (bitwise round-to-nearest-even) and a binary16-shaped one below 2^15 (the conversion).
+template <bool F16>
+__device__ __forceinline__ float leaf_rto(float p, float na, const GemmArgs& g) {
+#pragma clang fp contract(off)
+    const float t = na - p;
+    const float bb = t - na;
+    const float e = (na - (t - bb)) + (-p - bb);
+    const uint32_t tb = __float_as_uint(t);
+    const bool same = (int32_t)(__float_as_uint(e) ^ tb) >= 0;
+    const uint32_t odd = (same ? tb : tb - 1u) | 1u;
+    return leaf_exact<F16>(__uint_as_float(e == 0.0f ? tb : odd), g);
+}
+
 // Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
 // results are committed where valid, outputs needing the general path are recorded in a bit
 // mask, and one wave-uniform branch runs the general path for them (acc still holds their
@@ -498,8 +517,15 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // and the leaf formula applies when S <= sum_ehi (no top binade), Lf >= -126 (s is
         // zero or FP32-normal) and sL >= q (sums in the format's subnormal range are on its
         // grid).
-        bool sum_ok = false;
-        if (exact && sfin && g.range_f != 0u && sL >= g.sum_lreq && !g.sum_off) {
+        //
+        // Where that fails, RTO leaves (leaf_rto) still apply to a lane whose accumulator format
+        // they round exactly (mantissa-only, or binary16-shaped), when the slice's products are
+        // exact (chunk 1) or its chunk sums are FP32 values, nothing is inf / NaN and every sum
+        // stays below 2^sum_ehi.  The bound allows the RTO24 steps (a 2^-16 margin) and, for a
+        // chain's inexact chunk sums, |p| < 2^(sTop + 1).
+        bool sum_ok = false, rto_ok = false;
+        if (sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
+            (g.sum_off & 3) != 3) {
             uint32_t amax = 0u, mnk = 0xFFFFFFFFu;
 #pragma unroll
             for (int bi = 0; bi < WMB; ++bi)
@@ -516,20 +542,24 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                                     ? max(g.sum_q, (int)((mnk + 1u) >> 23) - 127 - g.sum_m)
                                     : 1000;
             const int Lf = min(sL, lsb_acc);
+            const int pt = exact ? sTop : sTop + 1;
             const float pa = ea > -1000 ? __builtin_ldexpf(1.0f, max(ea, -126)) : 0.0f;
-            const float pp = sTop > -1000 ? __builtin_ldexpf(1.0f, max(sTop, -126)) : 0.0f;
+            const float pp = sTop > -1000 ? __builtin_ldexpf(1.0f, max(pt, -126)) : 0.0f;
             const float bnd = __builtin_fmaf(g.sum_g, __builtin_fmaf(g.sum_n, pp, pa), pp) *
-                              1.00000095367431640625f;   // 1 + 2^-20
+                              1.0000152587890625f;   // 1 + 2^-16
             const int S = (int)(__float_as_uint(bnd) >> 23) - 126;
-            sum_ok = S <= min(sL + 24, g.sum_ehi) && sTop <= Lf + 23 && Lf >= -126;
+            sum_ok = exact && sL >= g.sum_lreq && S <= min(sL + 24, g.sum_ehi) &&
+                     sTop <= Lf + 23 && Lf >= -126 && !(g.sum_off & 1);
+            rto_ok = !sum_ok && g.sum_rto && S <= g.sum_ehi && !(g.sum_off & 2);
         }
 
         const int kmax = min(BK, g.K - k0);
         if constexpr (CHUNK == 1) {
-          set_form(sum_ok);
-          // nacc <- -round(a * b + acc): fma(-a, b, nacc) is -(a * b + acc) exactly
-          auto exact_c1 = [&](auto f16) {
-            constexpr bool F16 = decltype(f16)::value;
+          set_form(sum_ok || rto_ok);
+          // nacc <- -round(a * b + acc): exact sums (fma(-a, b, nacc) is -(a * b + acc)
+          // exactly) or RTO leaves on the exact product a * b
+          auto leaves_c1 = [&](auto f16, auto rto) {
+            constexpr bool F16 = decltype(f16)::value, RTO = decltype(rto)::value;
             for (int kk = 0; kk < kmax; ++kk) {
                 float4 a4[WMB][4];
                 float b[WNB];
@@ -547,16 +577,24 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
 #pragma unroll
                         for (int rq = 0; rq < 4; ++rq) {
                             float* na = &nacc[bi][bj][rq * 4];
-                            na[0] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].x, b[bj], na[0]), g);
-                            na[1] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].y, b[bj], na[1]), g);
-                            na[2] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].z, b[bj], na[2]), g);
-                            na[3] = leaf_exact<F16>(__builtin_fmaf(-a4[bi][rq].w, b[bj], na[3]), g);
+                            const float av[4] = {a4[bi][rq].x, a4[bi][rq].y, a4[bi][rq].z,
+                                                 a4[bi][rq].w};
+#pragma unroll
+                            for (int x = 0; x < 4; ++x) {
+                                if constexpr (RTO)
+                                    na[x] = leaf_rto<F16>(av[x] * b[bj], na[x], g);
+                                else
+                                    na[x] = leaf_exact<F16>(__builtin_fmaf(-av[x], b[bj], na[x]), g);
+                            }
                         }
             }
           };
           if (sum_ok) {
-            if (g.sum_f16) exact_c1(std::true_type{});
-            else exact_c1(std::false_type{});
+            if (g.sum_f16) leaves_c1(std::true_type{}, std::false_type{});
+            else leaves_c1(std::false_type{}, std::false_type{});
+          } else if (rto_ok) {
+            if (g.sum_f16) leaves_c1(std::true_type{}, std::true_type{});
+            else leaves_c1(std::false_type{}, std::true_type{});
           } else {
             float (&acc)[WMB][WNB][16] = nacc;
             // acc <- round(RTO_fma(a, b, acc)) for every k: float products where the slice's
@@ -605,7 +643,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             }
           }
         } else {
-        set_form(sum_ok);
+        set_form(sum_ok || rto_ok);
         float (&acc)[WMB][WNB][16] = nacc;
 #pragma unroll
         for (int q = 0; q < BK / CHUNK; ++q) {
@@ -694,6 +732,24 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
 #pragma unroll
                             for (int r = 0; r < 16; ++r)
                                 nacc[bi][bj][r] = leaf_exact<false>(nacc[bi][bj][r] - part[bi][bj][r], g);
+                }
+            } else if (rto_ok) {   // nacc <- -round(RTO24(p + acc)) for any in-range sum
+                if (g.sum_f16) {
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r)
+                                nacc[bi][bj][r] = leaf_rto<true>(part[bi][bj][r], nacc[bi][bj][r], g);
+                } else {
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r)
+                                nacc[bi][bj][r] = leaf_rto<false>(part[bi][bj][r], nacc[bi][bj][r], g);
                 }
             } else {
 #pragma unroll

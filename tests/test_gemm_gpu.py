@@ -286,6 +286,30 @@ def test_exact_sum_transitions(dev, chunk):
                         f"transitions chunk {chunk} config {cfg} without exact-sum leaves")
 
 
+@pytest.mark.parametrize("chunk", [1, 2, 8, 32])
+def test_rto_leaves(dev, chunk):
+    """Inexact sums on the RTO leaves: bf16-like 8e7m operands of N(0, 1/64) (16-bit
+    products, exact in FP32; tiny values make acc + p inexact), with mantissa-only and
+    binary16-shaped accumulators, against the oracle; then the same tiles with the RTO leaves
+    off (0x1000) and with both fast leaf kinds off (0x1800)."""
+    import lofloat_amd as lof
+    g = np.random.default_rng(100 + chunk)
+    M, N, K = 96, 80, 320
+    A = oracle.virtual_round((g.standard_normal((M, K)) / 64).astype(np.float32),
+                             BF16_LIKE).reshape(M, K)
+    B = oracle.virtual_round((g.standard_normal((K, N)) / 64).astype(np.float32),
+                             BF16_LIKE).reshape(K, N)
+    A[3, :40] = 0.0                                          # zero sums at the start of a row
+    B[:, 5] *= np.float32(2.0 ** -20)                        # a column of tiny products
+    for acc, oacc in (_accums(lof, 10, chunk), _accums(lof, 7, chunk), _accums(lof, 19, chunk),
+                      _accums(lof, fmt=FP16_SAT, chunk=chunk), _accums(lof, fmt=FP16_EXT, chunk=chunk)):
+        ref = _ref(A, B, oacc)
+        for cfg in MFMA_CONFIGS:
+            for bits in (0, 0x1000, 0x1800):
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
+                            f"rto chunk {chunk} config {cfg} bits {bits:#x}")
+
+
 def test_alpha_and_reference_entry(dev):
     import lofloat_amd as lof
     A, B = _mats(64, 48, 96, seed=2)
