@@ -506,9 +506,10 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // Exact-sum test, per lane: is every sum s = acc + p of this slice exact in FP32?
         // With |acc| < 2^ea and |p| < 2^sTop at the slice start, a leaf grows |acc| to at
         // most (|acc| + 2^sTop)(1 + 2^-(m+1)), so over the slice's n leaves every s is below
-        // g (2^ea + n 2^sTop) + 2^sTop < 2^S, g = (1 + 2^-(m+1))^n (rounded up on the host,
-        // and the float evaluation given a 2^-20 margin).  Every acc is a format value, a multiple of
-        // 2^max(E(acc) - m, q), and at the start a multiple of 2^Lf, Lf = min(sL, lowest
+        // g (2^ea + n 2^sTop) + 2^sTop < 2^S, g = (1 + 2^-(m+1))^n (rounded up on the host;
+        // the float evaluation gets a 2^-16 margin, which also covers the RTO leaves below).
+        // Every acc is a format value, a multiple of 2^max(E(acc) - m, q), and at the start a
+        // multiple of 2^Lf, Lf = min(sL, lowest
         // such bound over the lane's nonzero acc); an exact sum keeps both properties, so by
         // induction every s is exact when
         //   S <= sL + 24       (acc a multiple of 2^sL: s is one too, with < 24 bits)
@@ -521,8 +522,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // Where that fails, RTO leaves (leaf_rto) still apply to a lane whose accumulator format
         // they round exactly (mantissa-only, or binary16-shaped), when the slice's products are
         // exact (chunk 1) or its chunk sums are FP32 values, nothing is inf / NaN and every sum
-        // stays below 2^sum_ehi.  The bound allows the RTO24 steps (a 2^-16 margin) and, for a
-        // chain's inexact chunk sums, |p| < 2^(sTop + 1).
+        // stays below 2^sum_ehi.  The margin allows the RTO24 steps ((1 + 2^-23)^n) and, for a
+        // chain's inexact chunk sums, the bound takes |p| < 2^(sTop + 1).
         bool sum_ok = false, rto_ok = false;
         if (sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
             (g.sum_off & 3) != 3) {
