@@ -1,0 +1,189 @@
+"""CPU checks of the arithmetic behind the MFMA tile's two short leaves, against the oracle's
+reference leaf acc <- round(RTO_fma(p, 1, acc)) (cutlass_gemms.cuh:73-88, 139-165):
+
+- exact-sum leaves: the per-slice, per-lane test (slice statistics of the products, the
+  lane's accumulators, the growth bound) is re-derived here in numpy.  In every slice that
+  passes, every sum acc + p must be exact in FP32, and the 3-op integer round-to-nearest-even
+  of -(acc + p) must give the reference leaf, negated.
+- RTO leaves: from t = RN24(nacc - p) and its TwoSum residual, the rebuilt round-to-odd value,
+  rounded by the integer RNE (mantissa-only accumulators) or by binary16 conversion
+  (binary16-shaped ones), must equal the reference leaf for any in-range sum.
+
+Random formats, scales, start accumulators and chunks are drawn from a fixed seed, and the
+sizes are kept small so the file runs in seconds."""
+import math
+
+import numpy as np
+
+import oracle
+
+F32 = np.float32
+E4M3 = (8, 3, 7, 1, 0, 0)
+
+
+def _bits(x):
+    return int(np.array([x], np.float32).view(np.uint32)[0])
+
+
+def _float(b):
+    return np.array([b & 0xFFFFFFFF], np.uint32).view(np.float32)[0]
+
+
+def _rne(b, rf):
+    """Round-to-nearest-even of a float pattern at bit rf, sign kept."""
+    return (b + (1 << (rf - 1)) - 1 + ((b >> rf) & 1)) & ~((1 << rf) - 1) & 0xFFFFFFFF
+
+
+def _ref_leaf(p, acc, fmt, m):
+    x = oracle.rto_fma(float(p), 1.0, float(acc))
+    return F32(oracle.virtual_round_mant_scalar(x, m) if fmt is None
+               else oracle.virtual_round_scalar(x, fmt))
+
+
+def _slice_stats(v, key):
+    """amax exponent, smallest-nonzero exponent check, widest significand and lowest-bit
+    bound of a slice operand, as the tile's staging gathers them."""
+    b = np.asarray(v, np.float32).ravel().view(np.uint32).astype(np.int64)
+    mag = b & 0x7FFFFFFF
+    if mag.max() == 0:
+        return None
+    mn = int(mag[mag != 0].min())
+    orb = int(np.bitwise_or.reduce(b))
+    o = (orb & 0x7FFFFF) | 0x800000
+    ctz = (o & -o).bit_length() - 1
+    lsb = (mn >> 23) - 150 + ctz
+    if key:   # per value: E + min(ctz, 24), E from (|v| << 1) - 1
+        nz = ((mag << 1) - 1) & 0xFFFFFFFF
+        bb = b | 0x80000000
+        ctzs = np.array([min((int(x) & -int(x)).bit_length() - 1, 24) for x in bb])
+        lsb = max(lsb, int(((nz >> 24) + ctzs).min()) - 150)
+    return dict(emax=(int(mag.max()) >> 23) - 127, normal=(mn >> 23) != 0, width=24 - ctz,
+                lsb=lsb)
+
+
+def _growth(m, chunk):
+    n = 1 if chunk >= 32 else 32 // chunk
+    gr = (1 + 2.0 ** -(m + 1)) ** n
+    gf = F32(gr)
+    if float(gf) < gr:
+        gf = np.nextafter(gf, F32(1e30))
+    return gf, n
+
+
+def _bound_exponent(ea, sTop, m, chunk):
+    gf, n = _growth(m, chunk)
+    pa = F32(2.0 ** max(ea, -126)) if ea > -1000 else F32(0)
+    pp = F32(2.0 ** max(sTop, -126)) if sTop > -1000 else F32(0)
+    with np.errstate(over="ignore"):
+        inner = F32(np.float64(n) * np.float64(pp) + np.float64(pa))
+        bnd = F32(np.float64(gf) * np.float64(inner) + np.float64(pp))
+        bnd = F32(bnd * F32(1.0000152587890625))
+    return (_bits(bnd) >> 23) - 126
+
+
+def test_exact_sum_leaves():
+    g = np.random.default_rng(2024)
+    passed = checked = 0
+    for _ in range(60):
+        if g.random() < 0.3:
+            fmt, m, q, ehi = None, int(g.integers(1, 20)), -149, 127
+        else:
+            m, ew = int(g.integers(1, 20)), int(g.integers(2, 9))
+            bias = int(g.integers(-2, 2 ** (ew - 1) + 20))
+            fmt = (1 + ew + m, m, bias, 1, 0, 0)
+            q = max(1 - bias - m, -149)
+            ehi = max(min((1 << ew) - 1 - bias, 127), max(1 - bias, -126))
+        chunk = int(g.choice([1, 2, 4, 8, 16, 32]))
+        lanes, K = 8, 64
+        sc = 2.0 ** g.integers(-12, 6)
+        A = (g.standard_normal((lanes, K)) * sc * 2.0 ** g.integers(-3, 4, (lanes, 1))).astype(F32)
+        B = (g.standard_normal(K) * sc * np.where(g.random(K) < 0.1, 2.0 ** g.integers(-10, 6, K), 1)).astype(F32)
+        A = oracle.virtual_round(A.ravel(), E4M3).reshape(lanes, K)
+        B = oracle.virtual_round(B, E4M3)
+        A[g.random((lanes, K)) < 0.1] = 0
+        acc0 = (g.standard_normal(lanes) * sc * sc * 2.0 ** g.integers(-8, 10, lanes)).astype(F32)
+        acc0[g.random(lanes) < 0.2] = 0
+        acc = (oracle.virtual_round_mant(acc0, m) if fmt is None else oracle.virtual_round(acc0, fmt)).astype(F32)
+        if not np.all(np.isfinite(acc)):
+            continue
+        rf = 23 - m
+        for k0 in range(0, K, 32):
+            sa, sb = _slice_stats(A[:, k0:k0 + 32], chunk <= 4), _slice_stats(B[k0:k0 + 32], chunk <= 4)
+            if sa is None or sb is None:
+                exact, sL, sTop = True, 1000, -1000
+            else:
+                L = sa["lsb"] + sb["lsb"]
+                if chunk == 1:
+                    sTop = sa["emax"] + sb["emax"] + 2
+                    exact = (sa["normal"] and sb["normal"] and sa["width"] + sb["width"] <= 24
+                             and sTop <= 127 and L >= -149)
+                else:
+                    sTop = int(math.log2(chunk)) + sa["emax"] + sb["emax"] + 2
+                    exact = sa["normal"] and sb["normal"] and L >= -126 and sTop <= L + 24 and sTop <= 127
+                sL = L
+            ok = False
+            if exact and sL >= max(q, -126):
+                mag = acc.view(np.uint32).astype(np.int64) & 0x7FFFFFFF
+                ea = (int(mag.max()) >> 23) - 126 if mag.max() else -1000
+                nz = mag[mag != 0]
+                lsb_acc = max(q, (int(nz.min()) >> 23) - 127 - m) if nz.size else 1000
+                Lf = min(sL, lsb_acc)
+                S = _bound_exponent(ea, sTop, m, chunk)
+                ok = S <= min(sL + 24, ehi) and sTop <= Lf + 23 and Lf >= -126
+            passed += ok
+            for j in range(0, 32, chunk):
+                for i in range(lanes):
+                    p = F32(0.0)
+                    for k in range(k0 + j, k0 + j + chunk):   # the FP32 fmaf chain
+                        p = F32(np.float64(A[i, k]) * np.float64(B[k]) + np.float64(p))
+                    ref = _ref_leaf(p, acc[i], fmt, m)
+                    if ok:
+                        x = float(acc[i]) + float(p)
+                        assert float(F32(x)) == x, ("inexact sum in a passing slice", acc[i], p)
+                        t = F32(F32(-acc[i]) - p)
+                        got = -_float(_rne(_bits(t), rf))
+                        assert _bits(got) == _bits(ref), (fmt, m, chunk, acc[i], p, got, ref)
+                        checked += 1
+                    acc[i] = ref
+    assert passed > 10 and checked > 1000, (passed, checked)
+
+
+def test_rto_leaves():
+    g = np.random.default_rng(77)
+    checked = 0
+    for it in range(12):
+        f16 = it % 2 == 0
+        m = 10 if f16 else int(g.integers(1, 20))
+        fmt = (16, 10, 15, it % 4 // 2, 0, 0) if f16 else None
+        ehi = 15 if f16 else 127
+        rf = 23 - m
+        n = 600
+        sc = 2.0 ** g.uniform(-40 if f16 else -140, ehi - 3, n)
+        acc = (g.standard_normal(n) * sc).astype(F32)
+        acc = oracle.virtual_round_mant(acc, m) if fmt is None else oracle.virtual_round(acc, fmt)
+        a = oracle.virtual_round_mant(g.standard_normal(n).astype(F32), 7)
+        b = oracle.virtual_round_mant((g.standard_normal(n) * sc * 2.0 ** g.uniform(-40, 2, n)).astype(F32), 7)
+        p = (a.astype(np.float64) * b.astype(np.float64)).astype(F32)   # exact 16-bit products
+        p[g.random(n) < 0.05] = 0
+        acc[g.random(n) < 0.05] = 0
+        acc[(acc == 0) & (g.random(n) < 0.5)] = F32(-0.0)
+        for i in range(n):
+            ac, pv = F32(acc[i]), F32(p[i])
+            if abs(float(ac) + float(pv)) >= 2.0 ** ehi * 0.9:
+                continue
+            na = F32(-ac)
+            t = F32(na - pv)
+            bb = F32(t - na)
+            e = F32(F32(na - F32(t - bb)) + F32(F32(-pv) - bb))   # TwoSum residual
+            tb = _bits(t)
+            if e == 0:
+                rto = tb
+            elif (_bits(e) ^ tb) >> 31 == 0:
+                rto = tb | 1
+            else:
+                rto = (tb - 1) | 1
+            r = F32(np.float16(_float(rto))) if f16 else _float(_rne(rto, rf))
+            ref = _ref_leaf(pv, ac, fmt, m)
+            assert _bits(-r) == _bits(ref), (fmt, m, ac, pv, t, e, -r, ref)
+            checked += 1
+    assert checked > 5000
