@@ -104,15 +104,14 @@ TuneKey tune_key(const lof_gemm_desc& d) {
 // Untuned choice, from measurements at M = N = K = 4096 and on conv-shaped GEMMs
 // (M = 64..512, e4m3 / 8e7m inputs): wave tiles for the chunks they take, 64x128 (each
 // lane's A values serve two B columns) where that still gives enough tiles -- >= 384 (1.5
-// per CU) for c >= 8, >= 768 for chunk 1, whose exact-sum leaves are LDS-bound (e4m3 +10-15 %,
-// 8e7m on the RTO leaves -4-9 %) -- else 64x64; the 32x64 VALU tile for other chunks; the
-// 32x32 VALU tile when there are too few tiles to fill the chip.
+// per CU) for chunks 2..32 (+18-22 % over 64x64 at chunks 2 and 4), >= 768 for chunk 1,
+// whose exact-sum leaves are LDS-bound (e4m3 +10-15 %, 8e7m on the RTO leaves -4-9 %) --
+// else 64x64; the 32x64 VALU tile for other chunks; the 32x32 VALU tile when there are too
+// few tiles to fill the chip.
 int default_config(const lof_gemm_desc& d) {
     if (chunk_fits_wave_tile(d.accum.chunk)) {
         const int64_t tiles_64x128 = (int64_t)((d.M + 63) / 64) * ((d.N + 127) / 128);
-        if (d.accum.chunk >= 8 && tiles_64x128 >= 384) return kMfma64x128;
-        if (d.accum.chunk == 1 && tiles_64x128 >= 768) return kMfma64x128;
-        return kMfma64;
+        return tiles_64x128 >= (d.accum.chunk == 1 ? 768 : 384) ? kMfma64x128 : kMfma64;
     }
     if ((int64_t)((d.M + 31) / 32) * ((d.N + 63) / 64) >= 1024) return 3;
     return 4;
