@@ -238,21 +238,38 @@ def extras(lof, torch, np, oracle, dev):
     log(f"mx variants: {out['mx_e2m1_GBs']} / {out['mx_e4m3_srA8_bits_GBs']} GB/s")
     del x, y, bits
 
-    # C3: emulated GEMM 4096^3 on e4m3 / 8e7m pre-rounded inputs
+    # C3: emulated GEMM 4096^3 on e4m3 / 8e7m pre-rounded inputs.  Parity: 4096 sampled
+    # outputs (full K) against the oracle, bit for bit; the oracle's time on them gives the
+    # CPU rate in emulated multiply-adds per second.
     N = 4096
     flop = 2.0 * N ** 3
+    samples = np.random.default_rng(11).choice(N * N, 4096, replace=False)
+    sample_t = torch.from_numpy(samples).to(dev)
     for inp in ("e4m3", "8e7m"):
         A, B = gemm_inputs(lof, torch, dev, inp, N)
-        for name, acc in (("mant10", lambda c: lof.AccumSpec(mant_bits=10, chunk=c)),
-                          ("5e10m", lambda c: lof.AccumSpec(fmt=fp16, chunk=c))):
+        Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+        for name, acc, oacc in (
+                ("mant10", lambda c: lof.AccumSpec(mant_bits=10, chunk=c),
+                 lambda c: oracle.accum_mant(10, c)),
+                ("5e10m", lambda c: lof.AccumSpec(fmt=fp16, chunk=c),
+                 lambda c: oracle.accum_fmt(fp16.as_tuple(), c))):
             for chunk in (1, 8, 32):
                 spec = acc(chunk)
                 it = 2 if chunk == 1 else 4
                 ms = timed(lambda i: lof.gemm(A, B, spec), it)
-                key = f"gemm4096_{inp}_{name}_c{chunk}_TFLOPs"
-                out[key] = round(flop / ms / 1e9, 3)
-                log(f"{key}: {ms:.2f} ms, {out[key]} TFLOP/s")
-        del A, B
+                key = f"gemm4096_{inp}_{name}_c{chunk}"
+                out[key + "_TFLOPs"] = round(flop / ms / 1e9, 3)
+                got = lof.gemm(A, B, spec).flatten()[sample_t].cpu().numpy()
+                t0 = time.perf_counter()
+                ref = oracle.gemm(Ah, Bh, oacc(chunk), samples=samples)
+                cpu_s = time.perf_counter() - t0
+                same = bool(np.array_equal(got.view(np.uint32), np.asarray(ref, np.float32).ravel().view(np.uint32)))
+                out[key + "_parity_4096_samples"] = same
+                out[key + "_cpu_oracle_GMACs"] = round(samples.size * N / cpu_s / 1e9, 4)
+                log(f"{key}: {ms:.2f} ms, {out[key + '_TFLOPs']} TFLOP/s, sampled parity {same}, "
+                    f"oracle {out[key + '_cpu_oracle_GMACs']} G emulated MAC/s")
+        del A, B, Ah, Bh
+    out["gemm_cpu_oracle_threads"] = oracle.max_threads()
     out["gemm_roofline_peak_TFLOPs"] = VALU_FP32_PEAK_TFLOPS
 
     linear_steps(lof, torch, np, dev, out)
