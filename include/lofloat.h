@@ -172,12 +172,17 @@ typedef struct lof_gemm_desc {
 } lof_gemm_desc;
 
 /* Run one GEMM.  config: -1 = autotuned choice (or the default tile), else an index
- * < lof_gemm_num_configs() that is valid for the chunk (lof_gemm_config_valid); OR-ing
- * 0x100 forces the literal (general) leaf path, 0x200 makes an MFMA config compute every
- * chunk sum with the VALU fmaf chain, and 0x400 makes it compute every chunk sum on MFMA
- * without the exactness test (all three for testing; 0x400 can change results).  Chunk sums run on MFMA only where
- * every partial sum of the chunk is exact in FP32, checked per tile and k-slice at run time,
- * so all configs give bit-identical results.  Replaces lo_float::LoF_gemm
+ * < lof_gemm_num_configs() that is valid for the chunk (lof_gemm_config_valid).  Testing
+ * bits OR-ed into config:
+ *   0x100   forces the literal (general) leaf path;
+ *   0x200   makes an MFMA config compute every chunk sum with the VALU fmaf chain;
+ *   0x400   makes it compute every chunk sum on MFMA without the exactness test (this one
+ *           can change results);
+ *   0x800   turns the MFMA configs' exact-sum leaves off;
+ *   0x1000  turns their RTO leaves off.
+ * Chunk sums run on MFMA only where every partial sum of the chunk is exact in FP32, and the
+ * short leaves only where they provably equal the reference leaf, both checked per tile and
+ * k-slice at run time, so all configs give bit-identical results.  Replaces lo_float::LoF_gemm
  * (Lof_kernel.cu:96-154) and lo_float::Gemm::operator() (cutlass_gemms.cuh:1789-1822;
  * alpha = 1/(scale_a*scale_b) per run_scaled, 1777-1787, is the caller's job). */
 LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_stream_t stream);
