@@ -87,8 +87,9 @@ def main():
     ap.add_argument("--config", type=int, default=-1, help="GEMM tile config (-1: default)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards with no exchange: see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (rounding "
+              "shards by element ranges and the GEMM by output tiles, with no exchange)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
