@@ -455,6 +455,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // multiple of 2^sL and below 2^sTop in magnitude; sfin: no inf or NaN in the slice
         int sL = 0, sTop = 0;
         bool sfin = false;
+        // chunk 1: every nonzero B value of the slice is a multiple of 2^bLsb (so at least
+        // 2^bLsb) and below 2^(bTop + 1); left at values no test passes when a slice is zero
+        int bLsb = -1000, bTop = 1000;
         {
             uint32_t amA = 0u, mnA = 0xFFFFFFFFu, orA = 0u, amB = 0u, mnB = 0xFFFFFFFFu, orB = 0u;
             uint32_t kyA = 0xFFFFFFFFu, kyB = 0xFFFFFFFFu;
@@ -487,6 +490,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     const int wB = 24 - __builtin_ctz((orB & 0x7FFFFFu) | 0x800000u);
                     exact = normA && normB && wA + wB <= 24 && eA + eB + 2 <= 127 && L >= -149;
                     sTop = eA + eB + 2;
+                    bLsb = lsbB;
+                    bTop = eB;
                 } else {
                     const int top = LOG2C + eA + eB + 2;
                     exact = normA && normB && L >= -126 && top <= L + 24 && top <= 127;
@@ -524,7 +529,19 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // exact (chunk 1) or its chunk sums are FP32 values, nothing is inf / NaN and every sum
         // stays below 2^sum_ehi.  The margin allows the RTO24 steps ((1 + 2^-23)^n) and, for a
         // chain's inexact chunk sums, the bound takes |p| < 2^(sTop + 1).
-        bool sum_ok = false, rto_ok = false;
+        //
+        // Chunk 1 holds a passing lane's accumulators as binary16 values h = -acc 2^hs, so
+        // one mixed-precision FMA, h <- binary16(fma(-a, b 2^hs, h)), is the whole leaf (the
+        // FP32 fma is exact, so rounding it to binary16 is rounding the sum).  That is the
+        // accumulator's rounding, scaled by 2^hs, when every scaled sum is below 2^15 (no
+        // overflow) and a multiple of 2^-24 (binary16 keeps any such sum below 2^-14, which
+        // has at most 10 significant bits, so the accumulator keeps it too; from 2^-14 up both
+        // round to nearest-even at 11 significant bits):
+        //   binary16-shaped format: hs = 0 (its own grid; sum_ehi is 15);
+        //   10-bit mantissa-only:   hs = 15 - S, which needs Lf + hs >= -24, i.e.
+        //                           S - Lf <= 39, and b 2^hs FP32-normal for every nonzero b.
+        bool sum_ok = false, rto_ok = false, h_ok = false;
+        int hs = 0;
         if (sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
             (g.sum_off & 3) != 3) {
             uint32_t amax = 0u, mnk = 0xFFFFFFFFu;
@@ -552,6 +569,16 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             sum_ok = exact && sL >= g.sum_lreq && S <= min(sL + 24, g.sum_ehi) &&
                      sTop <= Lf + 23 && Lf >= -126 && !(g.sum_off & 1);
             rto_ok = !sum_ok && g.sum_rto && S <= g.sum_ehi && !(g.sum_off & 2);
+            if constexpr (CHUNK == 1) {
+                if (sum_ok && !(g.sum_off & 4)) {
+                    if (g.sum_f16) {
+                        h_ok = true;
+                    } else if (g.sum_h10) {
+                        hs = 15 - S;
+                        h_ok = S - Lf <= 39 && bLsb + hs >= -126 && bTop + 1 + hs <= 127;
+                    }
+                }
+            }
         }
 
         const int kmax = min(BK, g.K - k0);
@@ -590,7 +617,48 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         }
             }
           };
-          if (sum_ok) {
+          if (h_ok) {   // h <- binary16(fma(-a, b 2^hs, h)), h = nacc 2^hs (exact both ways)
+            _Float16 h[WMB][WNB][16];
+#pragma unroll
+            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        h[bi][bj][r] = (_Float16)__builtin_ldexpf(nacc[bi][bj][r], hs);
+            for (int kk = 0; kk < kmax; ++kk) {
+                float4 a4[WMB][4];
+                float b[WNB];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq)
+                        a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+                    b[bj] = __builtin_ldexpf(Bs[kk * LDB_S + bn0 + bj * 32 + l32], hs);
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                        for (int rq = 0; rq < 4; ++rq) {
+                            _Float16* hv = &h[bi][bj][rq * 4];
+                            const float av[4] = {a4[bi][rq].x, a4[bi][rq].y, a4[bi][rq].z,
+                                                 a4[bi][rq].w};
+#pragma unroll
+                            for (int x = 0; x < 4; ++x)
+                                hv[x] = (_Float16)__builtin_fmaf(-av[x], b[bj], (float)hv[x]);
+                        }
+            }
+#pragma unroll
+            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        nacc[bi][bj][r] = __builtin_ldexpf((float)h[bi][bj][r], -hs);
+          } else if (sum_ok) {
             if (g.sum_f16) leaves_c1(std::true_type{}, std::false_type{});
             else leaves_c1(std::false_type{}, std::false_type{});
           } else if (rto_ok) {

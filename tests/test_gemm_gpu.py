@@ -286,6 +286,54 @@ def test_exact_sum_transitions(dev, chunk):
                         f"transitions chunk {chunk} config {cfg} without exact-sum leaves")
 
 
+def _held_corner(seed, M=64, N=128, K=64):
+    """Rows 0/1 and 32/33 (one lane's accumulators): slice 0 leaves a large and a tiny
+    accumulator with bits 2^-10 below its exponent, slice 1's products lie between them, so
+    the exact-sum test passes with S - Lf near 40 for a 10-bit mantissa-only accumulator."""
+    g = np.random.default_rng(seed)
+    A = np.zeros((M, K), np.float32)
+    B = np.zeros((K, N), np.float32)
+    B[:32] = oracle.virtual_round((g.standard_normal((32, N)) / 64).astype(np.float32),
+                                  E4M3).reshape(32, N)
+    B[0] = 0.125
+    B[32:] = g.choice([0.125, 0.25, 0.375, -0.125, -0.25, -0.375], (32, N))
+    for r in (0, 32):
+        A[r, 0] = 2.0 ** int(g.integers(10, 14))
+        A[r + 1, 1:32] = oracle.virtual_round_mant(
+            (g.standard_normal(31) * 2.0 ** int(g.integers(-20, -14))).astype(np.float32), 3)
+        A[r:r + 2, 32:] = g.choice([1.0, 1.5, -1.0, -1.5], (2, 32)) * 2.0 ** int(g.integers(-9, -5))
+    return A, B
+
+
+@pytest.mark.parametrize("case", ["e4m3", "tiny", "rows", "corner"])
+def test_binary16_held_leaves(dev, case):
+    """Chunk 1's binary16-held leaves (the lane's accumulators held as binary16, scaled by
+    2^hs for a 10-bit mantissa-only accumulator) against the oracle, and the same tiles with
+    them off (0x2000): e4m3 operands of the GEMM workload; tiny ones (sums in binary16's
+    subnormal range); per-row, per-slice scales of 2^-20 .. 2^20 (lanes move between
+    passing and failing slices with varying hs); and lanes holding a large and a tiny
+    accumulator where S - Lf is near the test's limit of 39."""
+    import lofloat_amd as lof
+    g = np.random.default_rng(7)
+    if case == "corner":
+        A, B = _held_corner(11)
+    else:
+        M, N, K = 96, 144, 320
+        A, B = _mats(M, N, K, seed=31, kind="e4m3_small")
+        if case == "tiny":
+            A = oracle.virtual_round(A * np.float32(2.0 ** -4), E4M3).reshape(M, K)
+        elif case == "rows":
+            rs = np.repeat(2.0 ** g.integers(-20, 21, (M, K // 32)), 32, axis=1)
+            A = oracle.virtual_round_mant((A * rs).astype(np.float32), 3).reshape(M, K)
+    for acc, oacc in (_accums(lof, 10, 1), _accums(lof, fmt=FP16_SAT, chunk=1),
+                      _accums(lof, fmt=FP16_EXT, chunk=1)):
+        ref = _ref(A, B, oacc)
+        for cfg in MFMA_CONFIGS:
+            for bits in (0, 0x2000):
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
+                            f"held {case} config {cfg} bits {bits:#x}")
+
+
 @pytest.mark.parametrize("chunk", [1, 2, 8, 32])
 def test_rto_leaves(dev, chunk):
     """Inexact sums on the RTO leaves: bf16-like 8e7m operands of N(0, 1/64) (16-bit

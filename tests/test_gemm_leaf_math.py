@@ -187,3 +187,90 @@ def test_rto_leaves():
             assert _bits(-r) == _bits(ref), (fmt, m, ac, pv, t, e, -r, ref)
             checked += 1
     assert checked > 5000
+
+
+def test_binary16_held_leaves():
+    """Chunk 1's binary16-held leaves: a lane that passes the exact-sum test, and for a 10-bit
+    mantissa-only accumulator also S - Lf <= 39 with every scaled B value FP32-normal, holds
+    h = -acc 2^hs as binary16 (hs = 15 - S, or 0 for the binary16-shaped format) and takes
+    h <- binary16(-a (b 2^hs) + h), the sum exact, so binary16's one rounding of it.  Every
+    leaf -h 2^-hs must equal the reference leaf, and the held values must convert exactly
+    both ways.  Row scales spread the accumulators over wide exponent ranges (and into
+    binary16's subnormal range for the format), so lanes pass and fail both tests."""
+    g = np.random.default_rng(4242)
+    passed = checked = 0
+    for it in range(200):
+        fmt = None if it % 2 == 0 or it >= 160 else (16, 10, 15, it % 4 // 2, 0, 0)
+        m = 10
+        q = -149 if fmt is None else -24
+        ehi = 127 if fmt is None else 15
+        lanes = 8
+        if it < 160:
+            sc = 2.0 ** (g.integers(-30, 30) if fmt is None else g.integers(-9, 1))
+            K = 32 * int(g.integers(2, 6))
+            # per-slice row scales: lanes move between passing and failing slices
+            w = 24 if fmt is None else 6
+            rs = 2.0 ** g.integers(-w, w + 1, (lanes, K // 32))
+            rs[:, g.random(K // 32) < 0.4] = 1.0
+            rows = np.repeat(rs, 32, axis=1)
+            A = (g.standard_normal((lanes, K)) * sc * rows).astype(F32)
+            B = (g.standard_normal(K) / 8).astype(F32)
+            A = oracle.virtual_round(A.ravel(), E4M3).reshape(lanes, K) if fmt is not None else \
+                oracle.virtual_round_mant(A.ravel(), 3).reshape(lanes, K)
+            B = oracle.virtual_round(B, E4M3)
+            A[g.random((lanes, K)) < 0.1] = 0
+        else:
+            # slice 0 leaves row 0 with a large accumulator and row 1 with a tiny one that
+            # has bits down to 2^(E - 10); slice 1's products lie between, so the
+            # exact-sum test passes with S - Lf near 40 (then binary16 could not hold row 1)
+            K = 64
+            A = np.zeros((lanes, K), F32)
+            B = np.zeros(K, F32)
+            B[:32] = oracle.virtual_round((g.standard_normal(32) / 64).astype(F32), E4M3)
+            B[0] = 0.125
+            B[32:] = g.choice([0.125, 0.25, 0.375, -0.125, -0.25, -0.375], 32)
+            A[0, 0] = 2.0 ** int(g.integers(10, 14))
+            A[1, 1:32] = oracle.virtual_round_mant(
+                (g.standard_normal(31) * 2.0 ** int(g.integers(-20, -14))).astype(F32), 3)
+            A[:2, 32:] = g.choice([1.0, 1.5, -1.0, -1.5], (2, 32)) * 2.0 ** int(g.integers(-9, -5))
+        acc = np.zeros(lanes, F32)
+        for k0 in range(0, K, 32):
+            sa, sb = _slice_stats(A[:, k0:k0 + 32], True), _slice_stats(B[k0:k0 + 32], True)
+            ok = False
+            if sa is not None and sb is not None:
+                sL = sa["lsb"] + sb["lsb"]
+                sTop = sa["emax"] + sb["emax"] + 2
+                exact = (sa["normal"] and sb["normal"] and sa["width"] + sb["width"] <= 24
+                         and sTop <= 127 and sL >= -149)
+                if exact and sL >= max(q, -126):
+                    mag = acc.view(np.uint32).astype(np.int64) & 0x7FFFFFFF
+                    ea = (int(mag.max()) >> 23) - 126 if mag.max() else -1000
+                    nz = mag[mag != 0]
+                    lsb_acc = max(q, (int(nz.min()) >> 23) - 127 - m) if nz.size else 1000
+                    Lf = min(sL, lsb_acc)
+                    S = _bound_exponent(ea, sTop, m, 1)
+                    ok = S <= min(sL + 24, ehi) and sTop <= Lf + 23 and Lf >= -126
+                    if ok and fmt is None:
+                        hs = 15 - S
+                        ok = S - Lf <= 39 and sb["lsb"] + hs >= -126 and sb["emax"] + 1 + hs <= 127
+                    elif ok:
+                        hs = 0
+            passed += ok
+            if ok:
+                h = np.array([np.float16(math.ldexp(-float(v), hs)) for v in acc])
+                assert all(float(h[i]) == math.ldexp(-float(acc[i]), hs) for i in range(lanes))
+            for k in range(k0, k0 + 32):
+                for i in range(lanes):
+                    p = F32(np.float64(A[i, k]) * np.float64(B[k]))
+                    ref = _ref_leaf(p, acc[i], fmt, m)
+                    if ok:
+                        bs = F32(math.ldexp(float(B[k]), hs))
+                        assert bs == 0 or abs(float(bs)) >= 2.0 ** -126
+                        x = -float(A[i, k]) * float(bs) + float(h[i])   # exact in double
+                        assert float(F32(x)) == x, ("inexact scaled sum", acc[i], p, hs)
+                        h[i] = np.float16(x)
+                        got = F32(math.ldexp(-float(h[i]), -hs))
+                        assert _bits(got) == _bits(ref), (fmt, hs, acc[i], p, got, ref)
+                        checked += 1
+                    acc[i] = ref
+    assert passed > 10 and checked > 1000, (passed, checked)
