@@ -180,7 +180,8 @@ typedef struct lof_gemm_desc {
  *           can change results);
  *   0x800   turns the MFMA configs' exact-sum leaves off;
  *   0x1000  turns their RTO leaves off;
- *   0x2000  turns their binary16-held chunk-1 leaves off.
+ *   0x2000  turns their binary16-held chunk-1 leaves off;
+ *   0x4000  turns off only the packed binary16 form of those leaves.
  * Chunk sums run on MFMA only where every partial sum of the chunk is exact in FP32, and the
  * short leaves only where they provably equal the reference leaf, both checked per tile and
  * k-slice at run time, so all configs give bit-identical results.  Replaces lo_float::LoF_gemm

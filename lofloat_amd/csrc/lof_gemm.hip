@@ -295,10 +295,12 @@ LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_s
     // testing bits of config: 0x100 forces the general leaf path; 0x200 runs every chunk of
     // an MFMA config on the VALU chain; 0x400 runs every chunk on MFMA without the exactness
     // test (results then differ where a chunk is inexact); 0x800 turns the MFMA tile's
-    // exact-sum leaves off, 0x1000 its RTO leaves, 0x2000 its binary16-held chunk-1 leaves
+    // exact-sum leaves off, 0x1000 its RTO leaves, 0x2000 its binary16-held chunk-1 leaves,
+    // 0x4000 only their packed binary16 form
     if (config >= 0 && (config & 0x800)) g.sum_off |= 1;
     if (config >= 0 && (config & 0x1000)) g.sum_off |= 2;
     if (config >= 0 && (config & 0x2000)) g.sum_off |= 4;
+    if (config >= 0 && (config & 0x4000)) g.sum_off |= 8;
     if (config >= 0 && (config & 0x100)) set_fast_path(&g, false, 0);
     if (config >= 0 && (config & 0x200)) g.mfma_mode = 1;
     else if (config >= 0 && (config & 0x400)) g.mfma_mode = 2;

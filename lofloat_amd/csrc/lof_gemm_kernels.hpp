@@ -112,7 +112,7 @@ struct GemmArgs {
     int32_t sum_rto;     // RTO leaves round this accumulator exactly (mantissa-only, binary16)
     int32_t sum_h10;     // mantissa-only with m = 10: binary16's significand, any exponent
     int32_t sum_off;     // testing: bit 0 exact-sum leaves off, bit 1 RTO leaves off,
-                         // bit 2 binary16-held chunk-1 leaves off
+                         // bit 2 binary16-held chunk-1 leaves off, bit 3 their packed form
     int32_t vec;         // A, B 16-byte aligned with lda, ldb multiples of 4 (float4 loads)
     int32_t mfma_mode;   // MFMA tile: 0 exactness test, 1 every chunk on the VALU chain,
                          // 2 every chunk on MFMA (testing only: skips the test)

@@ -39,6 +39,7 @@
 namespace lofamd {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
@@ -456,8 +457,10 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         int sL = 0, sTop = 0;
         bool sfin = false;
         // chunk 1: every nonzero B value of the slice is a multiple of 2^bLsb (so at least
-        // 2^bLsb) and below 2^(bTop + 1); left at values no test passes when a slice is zero
-        int bLsb = -1000, bTop = 1000;
+        // 2^bLsb) and below 2^(bTop + 1), with at most bW significant bits; a16: every A
+        // value is a binary16 value; left at values no test passes when a slice is zero
+        int bLsb = -1000, bTop = 1000, bW = 99;
+        bool a16 = false;
         {
             uint32_t amA = 0u, mnA = 0xFFFFFFFFu, orA = 0u, amB = 0u, mnB = 0xFFFFFFFFu, orB = 0u;
             uint32_t kyA = 0xFFFFFFFFu, kyB = 0xFFFFFFFFu;
@@ -492,6 +495,10 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     sTop = eA + eB + 2;
                     bLsb = lsbB;
                     bTop = eB;
+                    bW = wB;
+                    // <= 11 significant bits, lowest bit >= 2^-24, below 2^16: binary16
+                    // holds it (normal from 2^-14, a multiple of 2^-24 below)
+                    a16 = wA <= 11 && lsbA >= -24 && eA <= 15;
                 } else {
                     const int top = LOG2C + eA + eB + 2;
                     exact = normA && normB && L >= -126 && top <= L + 24 && top <= 127;
@@ -540,7 +547,11 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         //   binary16-shaped format: hs = 0 (its own grid; sum_ehi is 15);
         //   10-bit mantissa-only:   hs = 15 - S, which needs Lf + hs >= -24, i.e.
         //                           S - Lf <= 39, and b 2^hs FP32-normal for every nonzero b.
-        bool sum_ok = false, rto_ok = false, h_ok = false;
+        //
+        // When every A value and every scaled B value is a binary16 value as well, the
+        // binary16 FMA h <- binary16(-a (b 2^hs) + h) computes the same leaf, two lanes'
+        // worth per packed instruction.
+        bool sum_ok = false, rto_ok = false, h_ok = false, pk_ok = false;
         int hs = 0;
         if (sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
             (g.sum_off & 3) != 3) {
@@ -577,6 +588,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         hs = 15 - S;
                         h_ok = S - Lf <= 39 && bLsb + hs >= -126 && bTop + 1 + hs <= 127;
                     }
+                    pk_ok = h_ok && a16 && bW <= 11 && bLsb + hs >= -24 && bTop + hs <= 15 &&
+                            !(g.sum_off & 8);
                 }
             }
         }
@@ -617,7 +630,55 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         }
             }
           };
-          if (h_ok) {   // h <- binary16(fma(-a, b 2^hs, h)), h = nacc 2^hs (exact both ways)
+          if (pk_ok) {   // pairs of rows: hp <- binary16 fma(-a, b 2^hs, hp), hp = nacc 2^hs
+            f16x2 hp[WMB][WNB][8];
+#pragma unroll
+            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 8; ++r)
+                        hp[bi][bj][r] = f16x2{(_Float16)__builtin_ldexpf(nacc[bi][bj][2 * r], hs),
+                                              (_Float16)__builtin_ldexpf(nacc[bi][bj][2 * r + 1], hs)};
+            for (int kk = 0; kk < kmax; ++kk) {
+                float4 a4[WMB][4];
+                f16x2 ah[WMB][8], bh[WNB];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq)
+                        a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj) {
+                    const _Float16 v =
+                        (_Float16)__builtin_ldexpf(Bs[kk * LDB_S + bn0 + bj * 32 + l32], hs);
+                    bh[bj] = f16x2{v, v};
+                }
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq) {
+                        ah[bi][2 * rq] = f16x2{(_Float16)-a4[bi][rq].x, (_Float16)-a4[bi][rq].y};
+                        ah[bi][2 * rq + 1] = f16x2{(_Float16)-a4[bi][rq].z, (_Float16)-a4[bi][rq].w};
+                    }
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                        for (int r = 0; r < 8; ++r)
+                            hp[bi][bj][r] = __builtin_elementwise_fma(ah[bi][r], bh[bj], hp[bi][bj][r]);
+            }
+#pragma unroll
+            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        nacc[bi][bj][2 * r] = __builtin_ldexpf((float)hp[bi][bj][r].x, -hs);
+                        nacc[bi][bj][2 * r + 1] = __builtin_ldexpf((float)hp[bi][bj][r].y, -hs);
+                    }
+          } else if (h_ok) {   // h <- binary16(fma(-a, b 2^hs, h)), h = nacc 2^hs (exact both ways)
             _Float16 h[WMB][WNB][16];
 #pragma unroll
             for (int bi = 0; bi < WMB; ++bi)

@@ -309,7 +309,7 @@ def _held_corner(seed, M=64, N=128, K=64):
 def test_binary16_held_leaves(dev, case):
     """Chunk 1's binary16-held leaves (the lane's accumulators held as binary16, scaled by
     2^hs for a 10-bit mantissa-only accumulator) against the oracle, and the same tiles with
-    them off (0x2000): e4m3 operands of the GEMM workload; tiny ones (sums in binary16's
+    only their packed binary16 form off (0x4000) and with them off (0x2000): e4m3 operands of the GEMM workload; tiny ones (sums in binary16's
     subnormal range); per-row, per-slice scales of 2^-20 .. 2^20 (lanes move between
     passing and failing slices with varying hs); and lanes holding a large and a tiny
     accumulator where S - Lf is near the test's limit of 39."""
@@ -329,7 +329,7 @@ def test_binary16_held_leaves(dev, case):
                       _accums(lof, fmt=FP16_EXT, chunk=1)):
         ref = _ref(A, B, oacc)
         for cfg in MFMA_CONFIGS:
-            for bits in (0, 0x2000):
+            for bits in (0, 0x4000, 0x2000):
                 assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
                             f"held {case} config {cfg} bits {bits:#x}")
 

@@ -193,12 +193,14 @@ def test_binary16_held_leaves():
     """Chunk 1's binary16-held leaves: a lane that passes the exact-sum test, and for a 10-bit
     mantissa-only accumulator also S - Lf <= 39 with every scaled B value FP32-normal, holds
     h = -acc 2^hs as binary16 (hs = 15 - S, or 0 for the binary16-shaped format) and takes
-    h <- binary16(-a (b 2^hs) + h), the sum exact, so binary16's one rounding of it.  Every
+    h <- binary16(-a (b 2^hs) + h), the sum exact, so binary16's one rounding of it (the
+    packed form, a binary16 FMA, needs every A and scaled B value to be a binary16 value:
+    checked here wherever its slice test passes).  Every
     leaf -h 2^-hs must equal the reference leaf, and the held values must convert exactly
     both ways.  Row scales spread the accumulators over wide exponent ranges (and into
     binary16's subnormal range for the format), so lanes pass and fail both tests."""
     g = np.random.default_rng(4242)
-    passed = checked = 0
+    passed = checked = packed = 0
     for it in range(200):
         fmt = None if it % 2 == 0 or it >= 160 else (16, 10, 15, it % 4 // 2, 0, 0)
         m = 10
@@ -255,7 +257,16 @@ def test_binary16_held_leaves():
                         ok = S - Lf <= 39 and sb["lsb"] + hs >= -126 and sb["emax"] + 1 + hs <= 127
                     elif ok:
                         hs = 0
+            # packed form: every A value and every scaled B value a binary16 value
+            pk = ok and (sa["width"] <= 11 and sa["lsb"] >= -24 and sa["emax"] <= 15 and
+                         sb["width"] <= 11 and sb["lsb"] + hs >= -24 and sb["emax"] + hs <= 15)
             passed += ok
+            packed += pk
+            if pk:
+                sl = A[:, k0:k0 + 32].ravel()
+                assert np.array_equal(sl.astype(np.float16).astype(F32), sl)
+                bsl = np.array([math.ldexp(float(v), hs) for v in B[k0:k0 + 32]])
+                assert np.array_equal(bsl.astype(np.float16).astype(np.float64), bsl)
             if ok:
                 h = np.array([np.float16(math.ldexp(-float(v), hs)) for v in acc])
                 assert all(float(h[i]) == math.ldexp(-float(acc[i]), hs) for i in range(lanes))
@@ -273,4 +284,4 @@ def test_binary16_held_leaves():
                         assert _bits(got) == _bits(ref), (fmt, hs, acc[i], p, got, ref)
                         checked += 1
                     acc[i] = ref
-    assert passed > 10 and checked > 1000, (passed, checked)
+    assert passed > 10 and checked > 1000 and 0 < packed < passed, (passed, checked, packed)
