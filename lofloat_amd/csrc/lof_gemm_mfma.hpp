@@ -549,7 +549,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         //                           S - Lf <= 39, and b 2^hs FP32-normal for every nonzero b.
         //
         // When every A value and every scaled B value is a binary16 value as well, the
-        // binary16 FMA h <- binary16(-a (b 2^hs) + h) computes the same leaf, two lanes'
+        // binary16 FMA h <- binary16(-a (b 2^hs) + h) computes the same leaf, two outputs'
         // worth per packed instruction.
         bool sum_ok = false, rto_ok = false, h_ok = false, pk_ok = false;
         int hs = 0;
@@ -590,6 +590,14 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     }
                     pk_ok = h_ok && a16 && bW <= 11 && bLsb + hs >= -24 && bTop + hs <= 15 &&
                             !(g.sum_off & 8);
+                }
+                // binary16-shaped format on RTO lanes (sums below 2^15, maybe inexact in
+                // FP32): the binary16 FMA rounds the exact sum once, which is what rounding
+                // the reference's RTO24 sum to 11 significant bits gives
+                if (rto_ok && g.sum_f16 && a16 && bW <= 11 && bLsb >= -24 && bTop <= 15 &&
+                    !(g.sum_off & 12)) {
+                    pk_ok = true;
+                    hs = 0;
                 }
             }
         }

@@ -334,6 +334,32 @@ def test_binary16_held_leaves(dev, case):
                             f"held {case} config {cfg} bits {bits:#x}")
 
 
+def test_packed_rto_leaves(dev):
+    """Chunk 1's packed binary16 FMA on RTO lanes (binary16-shaped accumulators, sums
+    inexact in FP32): bf16-like 8e7m operands of N(0, 1/64) with every value binary16 holds
+    (magnitudes below 2^-16 zeroed), except one tiny value in k 96..127 of A and of B, whose
+    slices then take the RTO leaves; against the oracle, and with the packed form off
+    (0x4000) and with every short leaf off (0x1800)."""
+    import lofloat_amd as lof
+    g = np.random.default_rng(808)
+    M, N, K = 96, 144, 320
+    A = oracle.virtual_round((g.standard_normal((M, K)) / 64).astype(np.float32),
+                             BF16_LIKE).reshape(M, K)
+    B = oracle.virtual_round((g.standard_normal((K, N)) / 64).astype(np.float32),
+                             BF16_LIKE).reshape(K, N)
+    A[np.abs(A) < 2.0 ** -16] = 0.0
+    B[np.abs(B) < 2.0 ** -16] = 0.0
+    assert np.array_equal(A.astype(np.float16).astype(np.float32), A)
+    A[5, 100] = np.float32(1.5 * 2.0 ** -30)
+    B[110, 7] = np.float32(1.25 * 2.0 ** -28)
+    for acc, oacc in (_accums(lof, fmt=FP16_SAT, chunk=1), _accums(lof, fmt=FP16_EXT, chunk=1)):
+        ref = _ref(A, B, oacc)
+        for cfg in MFMA_CONFIGS:
+            for bits in (0, 0x4000, 0x1800):
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
+                            f"packed rto config {cfg} bits {bits:#x}")
+
+
 @pytest.mark.parametrize("chunk", [1, 2, 8, 32])
 def test_rto_leaves(dev, chunk):
     """Inexact sums on the RTO leaves: bf16-like 8e7m operands of N(0, 1/64) (16-bit

@@ -285,3 +285,67 @@ def test_binary16_held_leaves():
                         checked += 1
                     acc[i] = ref
     assert passed > 10 and checked > 1000 and 0 < packed < passed, (passed, checked, packed)
+
+
+def _round_binary16(x):
+    """Round-to-nearest-even of an exact rational to IEEE binary16 (subnormals kept;
+    |x| < 2^15 here, so no overflow)."""
+    from fractions import Fraction
+    if x == 0:
+        return 0.0
+    mag = abs(x)
+    e = math.floor(math.log2(mag))
+    while Fraction(2) ** e > mag:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= mag:
+        e += 1
+    ulp = Fraction(2) ** (max(e, -14) - 10)
+    q, r = divmod(mag, ulp)
+    if r * 2 > ulp or (r * 2 == ulp and q % 2 == 1):
+        q += 1
+    v = float(q * ulp)
+    return v if x > 0 else -v
+
+
+def test_packed_binary16_rto_leaves():
+    """The packed binary16 FMA on RTO lanes of a binary16-shaped accumulator: with a, b and
+    acc binary16 values and |acc + a b| below 2^15, binary16's single rounding of the exact
+    -(a b + acc) (computed here in exact rational arithmetic), negated, must equal the
+    reference leaf round(RTO_fma(a b, 1, acc)), inexact FP32 sums included."""
+    from fractions import Fraction
+    g = np.random.default_rng(515)
+    checked = inexact = 0
+    for it in range(1600):
+        fmt = (16, 10, 15, it % 2, 0, 0)
+        # 8e7m-like operands that binary16 holds: 8 significant bits, |v| in [2^-16, 2^7]
+        a = F32(np.float16(oracle.virtual_round_mant(
+            np.array([g.standard_normal() * 2.0 ** g.integers(-16, 4)], F32), 7)[0]))
+        b = F32(np.float16(oracle.virtual_round_mant(
+            np.array([g.standard_normal() * 2.0 ** g.integers(-16, 4)], F32), 7)[0]))
+        acc = F32(oracle.virtual_round(np.array([g.standard_normal() * 2.0 ** g.integers(-24, 10)],
+                                                F32), fmt)[0])
+        r = g.random()
+        if r < 0.15:
+            acc = F32(-0.0) if g.random() < 0.5 else F32(0.0)
+        elif r < 0.2:
+            acc = F32(np.float16(-np.float64(a) * np.float64(b)))   # exact cancellations
+        if g.random() < 0.1:
+            a = F32(-0.0) if g.random() < 0.5 else F32(0.0)
+        p = F32(np.float64(a) * np.float64(b))
+        assert float(p) == float(a) * float(b)          # 16-bit products are exact
+        exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(acc))
+        if abs(exact) >= 2 ** 15:
+            continue
+        if exact == 0:
+            # the FMA's t = (-a) b + nacc, nacc = -acc: round-to-nearest gives +0 unless both
+            # terms are -0 (nonzero terms that cancel give +0)
+            neg = lambda x: math.copysign(1.0, float(x)) < 0
+            t_neg = float(p) == 0 and float(acc) == 0 and neg(a) == neg(b) and not neg(acc)
+            got = F32(0.0) if t_neg else F32(-0.0)
+        else:
+            inexact += float(F32(float(exact))) != exact
+            got = F32(-_round_binary16(-exact))
+        ref = _ref_leaf(p, acc, fmt, 10)
+        assert _bits(got) == _bits(ref), (a, b, acc, got, ref)
+        checked += 1
+    assert checked > 1200 and inexact > 120, (checked, inexact)
