@@ -638,7 +638,14 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         }
             }
           };
-          if (pk_ok) {   // pairs of rows: hp <- binary16 fma(-a, b 2^hs, hp), hp = nacc 2^hs
+          // pairs of rows: hp <- binary16 fma(-a, b 2^hs, hp), hp = nacc 2^hs; SCALED false
+          // for the binary16-shaped format, whose hs is always 0
+          auto pk_leaves = [&](auto scaled) {
+            constexpr bool SCALED = decltype(scaled)::value;
+            auto sc = [&](float v, int e) {
+                if constexpr (SCALED) return __builtin_ldexpf(v, e);
+                else return v;
+            };
             f16x2 hp[WMB][WNB][8];
 #pragma unroll
             for (int bi = 0; bi < WMB; ++bi)
@@ -646,8 +653,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 for (int bj = 0; bj < WNB; ++bj)
 #pragma unroll
                     for (int r = 0; r < 8; ++r)
-                        hp[bi][bj][r] = f16x2{(_Float16)__builtin_ldexpf(nacc[bi][bj][2 * r], hs),
-                                              (_Float16)__builtin_ldexpf(nacc[bi][bj][2 * r + 1], hs)};
+                        hp[bi][bj][r] = f16x2{(_Float16)sc(nacc[bi][bj][2 * r], hs),
+                                              (_Float16)sc(nacc[bi][bj][2 * r + 1], hs)};
             for (int kk = 0; kk < kmax; ++kk) {
                 float4 a4[WMB][4];
                 f16x2 ah[WMB][8], bh[WNB];
@@ -658,8 +665,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
 #pragma unroll
                 for (int bj = 0; bj < WNB; ++bj) {
-                    const _Float16 v =
-                        (_Float16)__builtin_ldexpf(Bs[kk * LDB_S + bn0 + bj * 32 + l32], hs);
+                    const _Float16 v = (_Float16)sc(Bs[kk * LDB_S + bn0 + bj * 32 + l32], hs);
                     bh[bj] = f16x2{v, v};
                 }
 #pragma unroll
@@ -683,9 +689,13 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 for (int bj = 0; bj < WNB; ++bj)
 #pragma unroll
                     for (int r = 0; r < 8; ++r) {
-                        nacc[bi][bj][2 * r] = __builtin_ldexpf((float)hp[bi][bj][r].x, -hs);
-                        nacc[bi][bj][2 * r + 1] = __builtin_ldexpf((float)hp[bi][bj][r].y, -hs);
+                        nacc[bi][bj][2 * r] = sc((float)hp[bi][bj][r].x, -hs);
+                        nacc[bi][bj][2 * r + 1] = sc((float)hp[bi][bj][r].y, -hs);
                     }
+          };
+          if (pk_ok) {
+            if (g.sum_f16) pk_leaves(std::false_type{});
+            else pk_leaves(std::true_type{});
           } else if (h_ok) {   // h <- binary16(fma(-a, b 2^hs, h)), h = nacc 2^hs (exact both ways)
             _Float16 h[WMB][WNB][16];
 #pragma unroll
