@@ -75,12 +75,14 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run N device copies of the 8192x8192 matrix (a plain copy "
                          "kernel of known traffic, to calibrate HBM byte counters)")
-    ap.add_argument("--only", choices=["gemm", "linear", "grouped", "pack", "conv"], default=None,
+    ap.add_argument("--only", choices=["gemm", "linear", "grouped", "pack", "conv", "round"],
+                    default=None,
                     help="run one workload alone (for profiling): 'gemm' = the 4096^3 "
                          "emulated-accumulation GEMM with --chunk / --accum / --inputs; "
                          "'linear' = the LoF_Linear step; 'grouped' = the batched launch; "
                          "'pack' = the bit-pack codec encode / decode of 2^28 elements; "
-                         "'conv' = the implicit-GEMM Conv2d")
+                         "'conv' = the implicit-GEMM Conv2d; 'round' = elementwise virtual_round "
+                         "of 2^24 / 2^28 values (with --no-cpu: no oracle check)")
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--accum", choices=["mant10", "5e10m"], default="5e10m")
     ap.add_argument("--inputs", choices=["e4m3", "8e7m"], default="e4m3")
@@ -104,6 +106,11 @@ def main():
     e8m0 = lof.create_e8m0_params()
     if args.only == "gemm":
         gemm_only(lof, torch, dev, args)
+        return
+    if args.only == "round":
+        out = {}
+        round_c1(lof, torch, np, dev, out, check=not args.no_cpu)
+        print(json.dumps(out), flush=True)
         return
     if args.only in ("linear", "grouped", "pack", "conv"):
         import numpy as np
@@ -187,6 +194,23 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def c1_device_values(torch, dev, n, seed, specials=0.01):
+    """N(0,1) * 2^U(-12,12) on the device, with about 1% of the entries replaced by +0, -0,
+    +inf, -inf, NaN or a random FP32 subnormal (equal shares)."""
+    g = torch.Generator(device=dev).manual_seed(seed)
+    x = torch.randn(n, device=dev, generator=g) * torch.exp2(
+        torch.rand(n, device=dev, generator=g) * 24 - 12)
+    k = int(n * specials)
+    idx = torch.randint(0, n, (k,), device=dev, generator=g)
+    kind = torch.randint(0, 6, (k,), device=dev, generator=g)
+    sub = (torch.randint(1, 1 << 23, (k,), device=dev, generator=g, dtype=torch.int32) |
+           (torch.randint(0, 2, (k,), device=dev, generator=g, dtype=torch.int32) << 31))
+    table = torch.tensor([0.0, -0.0, float("inf"), float("-inf"), float("nan")], device=dev)
+    vals = torch.where(kind < 5, table[kind.clamp(max=4)], sub.view(torch.float32))
+    x[idx] = vals
+    return x
+
+
 def gemm_inputs(lof, torch, dev, inputs, n=4096):
     fmt = lof.make_format(8, 3, 7) if inputs == "e4m3" else lof.make_format(16, 7, 127)
     g = torch.Generator(device=dev).manual_seed(3)
@@ -216,17 +240,7 @@ def extras(lof, torch, np, oracle, dev):
     e8m0 = lof.create_e8m0_params()
     fp16 = lof.make_format(16, 10, 15)
 
-    # C1: elementwise virtual_round, rotating buffers past the 256 MB MALL
-    for lg in (24, 28):
-        n = 1 << lg
-        nb = max(2, (1 << 30) // (8 * n) + 1)
-        xs = [torch.randn(n, device=dev) * torch.exp2(torch.rand(n, device=dev) * 24 - 12)
-              for _ in range(nb)]
-        ys = [torch.empty(n, device=dev) for _ in range(nb)]
-        ms = timed(lambda i: lof.virtual_round(xs[i % nb], e4m3, out=ys[i % nb]), 50 if lg == 24 else 10)
-        out[f"virtual_round_2^{lg}_e4m3_GBs"] = round(8 * n / ms / 1e6, 1)
-        log(f"virtual_round 2^{lg}: {ms * 1e3:.1f} us, {8 * n / ms / 1e6:.0f} GB/s")
-        del xs, ys
+    round_c1(lof, torch, np, dev, out)
     # C2 variants
     n = 8192 * 8192
     x = torch.randn(8192, 8192, device=dev)
@@ -278,6 +292,48 @@ def extras(lof, torch, np, oracle, dev):
     pack_codec(lof, torch, np, dev, out)
     conv_layer(lof, torch, np, dev, out)
     return out
+
+
+def round_c1(lof, torch, np, dev, out, check=True):
+    """Elementwise virtual_round (C1) at 2^24 and 2^28 values; ``check``: also compare every
+    output of one buffer with the oracle and time the oracle on it."""
+    import oracle
+    e4m3 = lof.make_format(8, 3, 7)
+    # C1: elementwise virtual_round of N(0,1) * 2^U(-12,12) with 1% +-0 / +-inf / NaN /
+    # FP32-subnormal specials to (8,3,7) RNE, rotating buffers past the 256 MB MALL; a plain
+    # device copy over the same buffers is the read+write stream ceiling it is compared with;
+    # every output of the first buffer is compared with the oracle bit for bit
+    for lg in (24, 28):
+        n = 1 << lg
+        nb = max(2, (1 << 30) // (8 * n) + 1)
+        xs = [c1_device_values(torch, dev, n, seed=100 + k) for k in range(nb)]
+        ys = [torch.empty(n, device=dev) for _ in range(nb)]
+        iters = 50 if lg == 24 else 10
+        ms = timed(lambda i: lof.virtual_round(xs[i % nb], e4m3, out=ys[i % nb]), iters)
+        ms_copy = timed(lambda i: ys[i % nb].copy_(xs[i % nb]), iters)
+        gbs, copy_gbs = 8 * n / ms / 1e6, 8 * n / ms_copy / 1e6
+        key = f"virtual_round_2^{lg}_e4m3"
+        out[key + "_GBs"] = round(gbs, 1)
+        out[key + "_frac_of_8TBs"] = round(gbs / HBM_PEAK_GBS, 4)
+        out[key + "_copy_ceiling_GBs"] = round(copy_gbs, 1)
+        out[key + "_frac_of_copy"] = round(gbs / copy_gbs, 4)
+        if not check:
+            log(f"virtual_round 2^{lg}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s (copy {copy_gbs:.0f})")
+            del xs, ys
+            continue
+        lof.virtual_round(xs[0], e4m3, out=ys[0])
+        xh = xs[0].cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oracle.virtual_round(xh, e4m3.as_tuple(), nthreads=oracle.max_threads())
+        cpu_s = time.perf_counter() - t0
+        got = ys[0].cpu().numpy()
+        same = bool(np.array_equal(got.view(np.uint32), ref.view(np.uint32)))
+        out[key + "_parity_full"] = same
+        out[key + "_cpu_oracle_GBs"] = round(8 * n / cpu_s / 1e9, 2)
+        log(f"virtual_round 2^{lg}: {ms * 1e3:.1f} us, {gbs:.0f} GB/s (copy {copy_gbs:.0f}), "
+            f"parity {same}, oracle {out[key + '_cpu_oracle_GBs']} GB/s")
+        del xs, ys, xh, ref, got
+        torch.cuda.empty_cache()
 
 
 def conv_layer(lof, torch, np, dev, out):

@@ -5,6 +5,8 @@
 //   enums + ProjSpec + FloatingPointParams      src/fp_tools.hpp:13-124, 187-252
 //   round_mantissa / round_fp_params             src/LoPy_bind.cpp:13-20, Lof_kernel.cu:37-60
 //   run_virtual_mx_round                          src/Lof_kernels.h:44-48, Lof_kernel.cu:70-94
+//   virtual_round over arrays (both overloads)    src/lo_float.h:4264-4268, 4276-4281
+//   virtual_mx_round                              src/lo_float.h:4311-4383
 //   Layout, Matrix<T, idx, Layout>                src/layouts.h:16-19, src/Matrix.h:21-78
 //   Gemm<MatA, MatB, MatC, MatD>                  src/cutlass_gemms.cuh:1629-1896
 //   Conv2d<...>, Conv2dProblemSize                src/cutlass_conv.cuh:20-184
@@ -133,6 +135,59 @@ void run_virtual_mx_round(float* inout, int64_t n, int block_size,
                                            (int32_t)ps_public.rounding_mode, rand_bits, seed,
                                            (lof_stream_t)stream),
                   "run_virtual_mx_round");
+}
+
+// lo_float::virtual_round over arrays (lo_float.h:4264-4268, mantissa-only overload;
+// 4276-4281, format overload), with the reference's argument order: values / results are
+// device arrays of n floats (they may be the same array).  The arithmetic is the scalar
+// virtual_round of every element (lo_float.h:2716-2756, 2844-2939), which the reference's
+// host SIMD array loop departs from (it drops signs and leaves a tail unwritten); stochastic
+// modes take one random word per element (rand_bits, or lof_rand_u32(seed, i)).
+template <typename From>
+void virtual_round(From* values, From* results, int ToMantissaBits, int n,
+                   ProjSpec ps = ProjSpec{}, hipStream_t stream = nullptr,
+                   const uint32_t* rand_bits = nullptr, uint64_t seed = 0) {
+    static_assert(std::is_same_v<From, float>, "the gfx950 path rounds FP32 arrays");
+    if (n <= 0) return;
+    detail::check(lof_virtual_round_mant_f32(values, results, n, ToMantissaBits,
+                                             detail::to_c(ps), 1.0f, rand_bits, seed,
+                                             (lof_stream_t)stream),
+                  "virtual_round");
+}
+
+template <typename From, class I, class N>
+void virtual_round(From* values, From* results, int n, FloatingPointParams<I, N> ToFp,
+                   ProjSpec ps = ProjSpec{}, hipStream_t stream = nullptr,
+                   const uint32_t* rand_bits = nullptr, uint64_t seed = 0) {
+    static_assert(std::is_same_v<From, float>, "the gfx950 path rounds FP32 arrays");
+    if (n <= 0) return;
+    const lof_format f = detail::to_c(ToFp);
+    detail::check(lof_virtual_round_f32(values, results, n, &f, detail::to_c(ps), 1.0f,
+                                        rand_bits, seed, (lof_stream_t)stream),
+                  "virtual_round");
+}
+
+// lo_float::virtual_mx_round (lo_float.h:4311-4383): in-place MX block rounding of a device
+// array of n floats in contiguous blocks of block_size (the last block may be short).  Each
+// block's scale is virtual_round(amax / max_normal(private), params_public) under
+// ps_public's mode with no random bits; elements round into params_private under
+// ps_private.  block_size must be a power of two in [4, 256] (else Error); n <= 0 or
+// block_size <= 0 does nothing, as in the reference.
+template <typename From, class PubInf, class PubNaN, class PrivInf, class PrivNaN>
+void virtual_mx_round(From* inout, int n, int block_size,
+                      FloatingPointParams<PubInf, PubNaN> params_public,
+                      FloatingPointParams<PrivInf, PrivNaN> params_private,
+                      ProjSpec ps_public = ProjSpec{}, ProjSpec ps_private = ProjSpec{},
+                      hipStream_t stream = nullptr, const uint32_t* rand_bits = nullptr,
+                      uint64_t seed = 0) {
+    static_assert(std::is_same_v<From, float>, "the gfx950 path rounds FP32 arrays");
+    if (block_size <= 0 || n <= 0) return;
+    const lof_format pub = detail::to_c(params_public), priv = detail::to_c(params_private);
+    detail::check(lof_virtual_mx_round_f32(inout, inout, n, block_size, &priv, &pub,
+                                           detail::to_c(ps_private),
+                                           (int32_t)ps_public.rounding_mode, rand_bits, seed,
+                                           (lof_stream_t)stream),
+                  "virtual_mx_round");
 }
 
 // Dense matrix view (Matrix.h:21-78): device pointer, m x n, leading dimension ld.

@@ -57,6 +57,13 @@ def _require_device_f32(t: torch.Tensor, what: str) -> None:
         raise RuntimeError(f"{what}: float32 tensors only (got {t.dtype})")
 
 
+def _require_same_device(t: torch.Tensor | None, ref: torch.Tensor, what: str) -> None:
+    """A caller-supplied output or source tensor must live on the input's device: its
+    pointer goes straight to a kernel on that device."""
+    if t is not None and t.device != ref.device:
+        raise ValueError(f"{what} must be on {ref.device} (got {t.device})")
+
+
 def _bits_ptr(rand_bits, ref: torch.Tensor, what: str):
     if rand_bits is None:
         return None, None
@@ -92,6 +99,7 @@ def virtual_round(input: torch.Tensor, params, round_mode=RoundingMode.RoundToNe
     writes ``out``, which may alias ``input``)."""
     _require_device_f32(input, "virtual_round")
     x = input.contiguous()
+    _require_same_device(out, x, "virtual_round: out")
     y = torch.empty_like(x) if out is None else out
     if y.shape != x.shape or y.dtype != torch.float32 or not y.is_contiguous():
         raise ValueError("virtual_round: out must be a contiguous float32 tensor of input's shape")
@@ -129,6 +137,7 @@ def virtual_mx_round(input: torch.Tensor, block_size: int, element_format: Float
     if int(block_size) <= 0:
         raise RuntimeError("virtual_mx_round: block_size must be positive")
     x = input.contiguous()
+    _require_same_device(out, x, "virtual_mx_round: out")
     y = torch.empty_like(x) if out is None else out
     if y.shape != x.shape or y.dtype != torch.float32 or not y.is_contiguous():
         raise ValueError("virtual_mx_round: out must be a contiguous float32 tensor")
@@ -208,6 +217,8 @@ def gemm(A: torch.Tensor, B: torch.Tensor, accum: AccumSpec, alpha: float = 1.0,
     N = B.shape[1]
     Av, lda, ta = _mat_view(A, "lof_gemm A")
     Bv, ldb, tb = _mat_view(B, "lof_gemm B")
+    _require_same_device(out, A, "lof_gemm: out")
+    _require_same_device(C if beta != 0.0 else None, A, "lof_gemm: C")
     D = torch.empty((M, N), dtype=torch.float32, device=A.device) if out is None else out
     if D.shape != (M, N) or not D.is_contiguous() or D.dtype != torch.float32:
         raise ValueError("lof_gemm: out must be a contiguous float32 (M, N) tensor")
@@ -289,6 +300,8 @@ def conv2d(x: torch.Tensor, w: torch.Tensor, accum: AccumSpec, stride=1, padding
     if P < 1 or Q < 1:
         raise RuntimeError(f"conv2d: non-positive output spatial size (P={P}, Q={Q})")
     shape = (N, P, Q, K) if nhwc else (N, K, P, Q)
+    _require_same_device(out, x, "conv2d: out")
+    _require_same_device(C if beta != 0.0 else None, x, "conv2d: C")
     y = torch.empty(shape, dtype=torch.float32, device=x.device) if out is None else out
     if tuple(y.shape) != shape or not y.is_contiguous() or y.dtype != torch.float32:
         raise ValueError(f"conv2d: out must be a contiguous float32 tensor of shape {shape}")

@@ -91,6 +91,8 @@ def decode(buf: torch.Tensor, params: FloatFormatDescriptor, count: int, endian=
     if buf.numel() < packed_size(params.total_bits, count):
         raise ValueError("pack.decode: buf is shorter than packed_size(bitwidth, count)")
     y = torch.empty(int(count), dtype=torch.float32, device=buf.device) if out is None else out
+    if y.device != buf.device:
+        raise ValueError(f"pack.decode: out must be on {buf.device} (got {y.device})")
     if y.dtype != torch.float32 or y.numel() != count or not y.is_contiguous():
         raise ValueError("pack.decode: out must be a contiguous float32 tensor of count elements")
     with torch.cuda.device(buf.device):

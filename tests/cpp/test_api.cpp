@@ -368,6 +368,16 @@ static void project_case(std::mt19937& rng) {
     (void)hipFree(dx);
 }
 
+static uint32_t* g_words = nullptr;
+static const uint32_t* dw_words(const std::vector<uint32_t>& w) {
+    if (!g_words) g_words = dev_copy(w);
+    return g_words;
+}
+static void free_words() {
+    (void)hipFree(g_words);
+    g_words = nullptr;
+}
+
 static void gpu_checks() {
     std::mt19937 rng(1234);
     const int n = 100003;
@@ -410,6 +420,44 @@ static void gpu_checks() {
     orc_virtual_mx_round(x.data(), want.data(), n, 32, &fe, &fs, orc_spec{0, 0, 0}, 0, nullptr, 0, 0);
     compare(host_copy(dm, n), want, "run_virtual_mx_round e2m1");
     (void)hipFree(d); (void)hipFree(dw); (void)hipFree(dout); (void)hipFree(din); (void)hipFree(dm);
+
+    // lo_float::virtual_round over arrays, format overload: e5m2 (8,2,15) saturating,
+    // round-to-odd, out of place
+    float* dv = dev_copy(x);
+    float* dr = dev_copy(std::vector<float>(n, 0.f));
+    FloatingPointParams<> e5m2(8, 2, 15, Saturating, _3109, Signed);
+    virtual_round(dv, dr, n, e5m2, ProjSpec(Rounding_Mode::RoundToOdd));
+    (void)hipDeviceSynchronize();
+    orc_format f52{8, 2, 15, 1, 0, 0};
+    orc_virtual_round_fmt_array(x.data(), want.data(), n, &f52, orc_spec{12, 0, 0}, 1.0f, nullptr,
+                                0, 0);
+    compare(host_copy(dr, n), want, "virtual_round(values, results, n, e5m2, RTO)");
+    // mantissa overload: 7 bits, SR-C with 6 random bits from explicit words, in place
+    float* di = dev_copy(x);
+    virtual_round(di, di, 7, n, ProjSpec(Rounding_Mode::StochasticRoundingC, OvfInf, 6), nullptr,
+                  dw_words(words));
+    (void)hipDeviceSynchronize();
+    orc_virtual_round_mant_array(x.data(), want.data(), n, 7, orc_spec{9, 0, 6}, 1.0f,
+                                 words.data(), 0, 0);
+    compare(host_copy(di, n), want, "virtual_round(values, results, 7, n, SR-C)");
+    // lo_float::virtual_mx_round: e4m3 elements RTZ, E8M0 scale RNE, bs 16, n not a
+    // multiple of the block (short last block)
+    FloatingPointParams<> e4m3s(8, 3, 7, Saturating, _3109, Signed);
+    float* dmx = dev_copy(x);
+    virtual_mx_round(dmx, n, 16, e8m0, e4m3s, ProjSpec{}, ProjSpec(Rounding_Mode::RoundTowardsZero));
+    (void)hipDeviceSynchronize();
+    orc_format f43{8, 3, 7, 1, 0, 0};
+    orc_virtual_mx_round(x.data(), want.data(), n, 16, &f43, &fs, orc_spec{1, 0, 0}, 0, nullptr, 0, 0);
+    compare(host_copy(dmx, n), want, "virtual_mx_round(inout, n, 16, e8m0, e4m3, RNE, RTZ)");
+    try {
+        virtual_mx_round(dmx, n, 24, e8m0, e4m3s);
+        EXPECT(false, "virtual_mx_round block size 24 must throw");
+    } catch (const Error& e) {
+        EXPECT(e.status == LOF_ERR_UNSUPPORTED, "status for block size 24");
+    }
+    virtual_mx_round(dmx, 0, 16, e8m0, e4m3s);   // n <= 0: nothing
+    (void)hipFree(dv); (void)hipFree(dr); (void)hipFree(di); (void)hipFree(dmx);
+    free_words();
 
     conv_case(rng);
     project_case(rng);

@@ -234,3 +234,34 @@ def test_argument_errors(dev):
         lof.virtual_round(torch.ones(4), _fmt(FORMATS["e4m3_sat"]))  # CPU tensor
     with pytest.raises(RuntimeError):
         lof.virtual_round(x.double(), _fmt(FORMATS["e4m3_sat"]))
+
+
+def test_foreign_device_outputs_rejected(dev):
+    """A host tensor passed as ``out`` (or as a GEMM / conv source ``C``) is refused before
+    any launch: its pointer is not device memory."""
+    import lofloat_amd as lof
+    x = torch.ones(64, device=dev)
+    host = torch.empty(64)
+    with pytest.raises(ValueError, match="out must be on"):
+        lof.virtual_round(x, lof.create_half_params(), out=host)
+    with pytest.raises(ValueError, match="out must be on"):
+        lof.virtual_round(x, 10, out=host)
+    with pytest.raises(ValueError, match="out must be on"):
+        lof.virtual_mx_round(x, 32, lof.create_p3109_params(8, 4), lof.create_e8m0_params(),
+                             out=host)
+    A = torch.ones(8, 8, device=dev)
+    acc = lof.AccumSpec(mant_bits=10)
+    with pytest.raises(ValueError, match="out must be on"):
+        lof.gemm(A, A, acc, out=torch.empty(8, 8))
+    with pytest.raises(ValueError, match="C must be on"):
+        lof.gemm(A, A, acc, C=torch.empty(8, 8), beta=1.0)
+    xc = torch.ones(1, 2, 5, 5, device=dev)
+    wc = torch.ones(3, 2, 3, 3, device=dev)
+    with pytest.raises(ValueError, match="out must be on"):
+        lof.conv2d(xc, wc, acc, out=torch.empty(1, 3, 3, 3))
+    with pytest.raises(ValueError, match="C must be on"):
+        lof.conv2d(xc, wc, acc, C=torch.empty(1, 3, 3, 3), beta=1.0)
+    fmt = lof.create_p3109_params(8, 4)
+    buf = lof.pack.encode(x, fmt)
+    with pytest.raises(ValueError, match="out must be on"):
+        lof.pack.decode(buf, fmt, 64, out=torch.empty(64))
