@@ -4,7 +4,8 @@ Drop-in names of the reference package ``LoFloat`` (LoFloat/__init__.py:1-5):
 ``virtual_round``, ``virtual_mx_round``, ``lof_gemm``, ``lof_conv2d``, ``pwl_silu``,
 ``RoundingMode``, ``InfBehavior``,
 ``NaNBehavior``, ``Signedness``, ``FloatFormatDescriptor``, the ``create_*_params``
-factories, the ``torch.ops.lofloat.*`` operators and the layers; ``lofloat_amd.pack`` is
+factories, the ``torch.ops.lofloat.*`` operators, the layers and the model helpers of
+LoFloat/utils.py; ``lofloat_amd.pack`` is
 the device bit-pack codec of lo_pack.hpp.  The arithmetic runs in
 hand-written HIP kernels in ``liblofloat_hip.so`` (C-ABI: include/lofloat.h); importing
 this package fails if that library has not been built.
@@ -25,6 +26,11 @@ from .layers import (ExplicitMultiheadAttention, FISRBatchNorm, L1BatchNorm,  # 
                      LinearGradSpec, LinfBatchNorm, LoF_Conv2d, LoF_Linear, LoF_Quantize,
                      LoFLinearFunction, PWLBatchNorm, PWLSiLU, STEMXRound, STERound, _fwht,
                      _quantize, exp_mant_quantize, mantissa_quantize, replace_mha_with_explicit)
+from . import utils  # noqa: F401,E402
+from .utils import (lofloatify, print_exp_mant, record_formats,  # noqa: F401,E402
+                    set_accumulation_precisions, set_all_to_3109, set_all_to_half,
+                    set_exponent_fields, set_exponentbias_fields, set_mantissa_fields,
+                    set_saturation_modes, sparsity_in_weights)
 from . import grouped  # noqa: F401,E402
 from . import pack  # noqa: F401,E402
 from .grouped import GemmJob, GroupedPlan, MXJob, RoundJob  # noqa: F401,E402

@@ -26,94 +26,120 @@ from .core import FloatFormatDescriptor, RoundingMode
 from .formats import create_e8m0_params, create_p3109_params, make_format
 
 
+def _rne_if_none(mode):
+    return RoundingMode.RoundToNearestEven if mode is None else mode
+
+
+def _round_into(tensor, params, mode, scale=1.0):
+    """scale * tensor rounded into ``params``: a registered format id (an int) goes through
+    the traceable torch.ops.lofloat.virtual_round, a descriptor through ops.virtual_round."""
+    if isinstance(params, int):
+        return torch.ops.lofloat.virtual_round(tensor, params, int(mode), 0, float(scale))
+    return ops.virtual_round(tensor, params, round_mode=mode, scale=float(scale))
+
+
+def _mx_round_into(tensor, element, scale_format, block_size, mode):
+    """MX rounding of contiguous blocks of the flattened tensor; registered ids take the
+    traceable op (scale rounded to nearest even, no stochastic bits)."""
+    if isinstance(element, int):
+        return torch.ops.lofloat.virtual_mx_round(tensor, int(block_size), element,
+                                                  scale_format, int(mode), 0, 0)
+    return ops.virtual_mx_round(tensor, int(block_size), element, scale_format, round_mode=mode)
+
+
 class STERound(torch.autograd.Function):
-    """round(scale * x) forward; backward multiplies by scale (layers.py:8-31)."""
+    """Fake quantisation y = round(scale * x); the straight-through gradient of the rounding
+    is the identity, so dy/dx = scale (layers.py:8-31)."""
 
     @staticmethod
     def forward(ctx, tensor, params, rounding_mode=RoundingMode.RoundToNearestEven, scale=1.0):
         ctx.scale = float(scale)
-        if isinstance(params, int):
-            return torch.ops.lofloat.virtual_round(tensor, params, int(rounding_mode), 0,
-                                                   float(scale))
-        return ops.virtual_round(tensor, params, round_mode=rounding_mode, stoch_len=0,
-                                 scale=float(scale))
+        return _round_into(tensor, params, rounding_mode, scale)
 
     @staticmethod
     def backward(ctx, grad_output):
-        if ctx.scale == 1.0:
-            return grad_output, None, None, None
-        return grad_output * ctx.scale, None, None, None
+        grad = grad_output if ctx.scale == 1.0 else grad_output * ctx.scale
+        return grad, None, None, None
 
 
 class STEMXRound(torch.autograd.Function):
-    """MX fake-quantisation along contiguous blocks of the last axis; identity backward
-    (layers.py:34-57)."""
+    """MX fake quantisation over blocks of the last axis; its output is in the input's own
+    range, so the straight-through gradient is the identity (layers.py:34-57)."""
 
     @staticmethod
     def forward(ctx, tensor, element_params, scale_params, block_size,
                 rounding_mode=RoundingMode.RoundToNearestEven):
-        if isinstance(element_params, int):
-            return torch.ops.lofloat.virtual_mx_round(
-                tensor, int(block_size), element_params, scale_params, int(rounding_mode),
-                int(RoundingMode.RoundToNearestEven), 0)
-        return ops.virtual_mx_round(tensor, int(block_size), element_params, scale_params,
-                                    round_mode=rounding_mode)
+        return _mx_round_into(tensor, element_params, scale_params, block_size, rounding_mode)
 
     @staticmethod
     def backward(ctx, grad_output):
-        return grad_output, None, None, None, None
+        return (grad_output,) + (None,) * 4
+
+
+_SCALINGS = ("per_tensor", "mx")
+
+
+def _check_scaling(scaling):
+    if scaling not in _SCALINGS:
+        raise ValueError(f"unknown scaling {scaling!r}: use one of {_SCALINGS}")
+
+
+def _check_blocks(length, block, what):
+    if length % block:
+        raise ValueError(f"MX blocks of {block} do not tile {what} of length {length}")
 
 
 class LoF_Quantize(nn.Module):
-    """Fake-quantise to a format, per tensor or MX along the last axis (layers.py:60-104)."""
+    """A fake-quantisation module: per tensor, or MX blocks along the last axis with a
+    shared ``scale_format`` scale (E8M0 by default) (layers.py:60-104)."""
 
     def __init__(self, params, rounding_mode=None, scaling="per_tensor", mx_block_size=32,
                  scale_format=None):
         super().__init__()
-        if scaling not in ("per_tensor", "mx"):
-            raise ValueError(f"scaling must be 'per_tensor' or 'mx', got {scaling!r}")
+        _check_scaling(scaling)
         self.params = params
-        self.rounding_mode = (rounding_mode if rounding_mode is not None
-                              else RoundingMode.RoundToNearestEven)
+        self.rounding_mode = _rne_if_none(rounding_mode)
         self.scaling = scaling
         self.mx_block_size = mx_block_size
-        self.scale_format = scale_format if scale_format is not None else create_e8m0_params()
+        self.scale_format = create_e8m0_params() if scale_format is None else scale_format
 
     def forward(self, x):
-        if self.scaling == "mx":
-            if x.shape[-1] % self.mx_block_size != 0:
-                raise ValueError(
-                    f"mx scaling needs the last dim ({x.shape[-1]}) divisible by "
-                    f"mx_block_size ({self.mx_block_size})")
-            return STEMXRound.apply(x, self.params, self.scale_format, self.mx_block_size,
-                                    self.rounding_mode)
-        return STERound.apply(x, self.params, self.rounding_mode)
+        if self.scaling == "per_tensor":
+            return STERound.apply(x, self.params, self.rounding_mode)
+        _check_blocks(x.shape[-1], self.mx_block_size, "the last axis")
+        return STEMXRound.apply(x, self.params, self.scale_format, self.mx_block_size,
+                                self.rounding_mode)
 
     def extra_repr(self):
-        s = f"params={self.params}, rounding_mode={self.rounding_mode}, scaling={self.scaling}"
+        parts = [f"params={self.params}", f"rounding_mode={self.rounding_mode}",
+                 f"scaling={self.scaling}"]
         if self.scaling == "mx":
-            s += f", mx_block_size={self.mx_block_size}"
-        return s
+            parts.append(f"mx_block_size={self.mx_block_size}")
+        return ", ".join(parts)
 
 
-def _quantize(self, tensor, params):
-    if params is None:
-        return tensor
-    return STERound.apply(tensor, params, self.rounding_mode)
+def _p3109_fields(exp_bits, mant_bits):
+    """The signed, saturating P3109 format with these exponent and mantissa field widths."""
+    return create_p3109_params(exp_bits + mant_bits + 1, mant_bits + 1, True, True)
+
+
+def _quantize(layer, tensor, params):
+    """Fake-quantise ``tensor`` to ``params`` under ``layer.rounding_mode``; no format, no
+    rounding (layers.py:106-109)."""
+    return tensor if params is None else STERound.apply(tensor, params, layer.rounding_mode)
 
 
 def mantissa_quantize(tensor, mantissa_bits):
-    if mantissa_bits is None:
-        return tensor
-    return STERound.apply(tensor, create_p3109_params(8 + mantissa_bits + 1, mantissa_bits + 1,
-                                                      True, True))
+    """Round to ``mantissa_bits`` with an 8-bit exponent field (layers.py:111-114)."""
+    return tensor if mantissa_bits is None else STERound.apply(
+        tensor, _p3109_fields(8, mantissa_bits))
 
 
 def exp_mant_quantize(tensor, exp_bits, mantissa_bits):
-    if exp_bits is None or mantissa_bits is None:
+    """Round to the given exponent / mantissa field widths (layers.py:116-119)."""
+    if None in (exp_bits, mantissa_bits):
         return tensor
-    return STERound.apply(tensor, create_p3109_params(exp_bits + mantissa_bits + 1,
-                                                      mantissa_bits + 1, True, True))
+    return STERound.apply(tensor, _p3109_fields(exp_bits, mantissa_bits))
 
 
 def _fwht(x: torch.Tensor, block_size=None) -> torch.Tensor:
@@ -198,72 +224,118 @@ class LoFLinearFunction(torch.autograd.Function):
         return dx, dw, None, None, None
 
 
-class _FormatSetters:
-    """set_mantissa / set_exponent / set_exponentbias / set_saturation_mode of the reference
-    layers (layers.py:756-783): each rebuilds the three P3109 formats from the current ones,
-    keeping the other fields."""
+# How each of the reference layers' format setters rebuilds a role's format from the old one
+# (layers.py:402-432, 757-787): rule(exp_bits, mant_bits, old, value) gives the arguments
+# (k, p, saturating, bias) of a signed P3109 format.  set_exponentbias and
+# set_saturation_mode hand the old mantissa_bits to create_p3109_params as its precision p,
+# so their rebuilt formats carry one mantissa bit fewer; that is kept as written.
+_FORMAT_RULES = {
+    "set_mantissa": lambda e, m, old, v: (1 + e + v, v + 1, True, None),
+    "set_exponent": lambda e, m, old, v: (1 + v + m, m + 1, True, None),
+    "set_exponentbias": lambda e, m, old, v: (old.total_bits, m, True, v),
+    "set_saturation_mode": lambda e, m, old, v: (old.total_bits, m, v, None),
+}
+_ROLES = ("act", "weight", "bias")
+
+
+def _role_format(params, exp_bits, mant_bits, role, required):
+    """A role's format: ``params`` if given, else the P3109 format of its field widths."""
+    if params is not None:
+        return params
+    if exp_bits is not None and mant_bits is not None:
+        return _p3109_fields(exp_bits, mant_bits)
+    if required:
+        raise ValueError(f"the {role} format needs {role}_params or both {role}_exp and "
+                         f"{role}_mant")
+    return None
+
+
+class _LoFLayer(nn.Module):
+    """What LoF_Linear and LoF_Conv2d share: the activation / weight / bias formats, the
+    rounding modes and scale factors, the accumulator and the scaling scheme, and the
+    reference layers' format setters."""
+
+    def _init_quantisation(self, *, formats, rounding_mode, scale_factors, accum_mant_bits,
+                           gemm_round_mode, stochastic_rounding_bits, scaling, mx_block_size,
+                           scale_format, hadamard, accum_format, accum_chunk):
+        _check_scaling(scaling)
+        self.rounding_mode = _rne_if_none(rounding_mode)
+        self.gemm_round_mode = _rne_if_none(gemm_round_mode)
+        self.act_scale_factor, self.w_scale_factor, self.b_scale_factor = scale_factors
+        self.accum_mant_bits = accum_mant_bits
+        self.stochastic_rounding_bits = stochastic_rounding_bits
+        self.accum_format = accum_format
+        self.accum_chunk = int(accum_chunk)
+        self.scaling = scaling
+        self.mx_block_size = mx_block_size
+        self.scale_format = create_e8m0_params() if scale_format is None else scale_format
+        self.hadamard_transform, self.hadamard_block_size = hadamard
+        for role, (params, e, m) in zip(_ROLES, formats):
+            setattr(self, f"{role}_params", _role_format(params, e, m, role, role != "bias"))
+        self._refresh_traced_ids()
+
+    def _refresh_traced_ids(self):
+        # the traceable ops take formats as registry ids and modes as ints
+        self._act_fid = register_format(self.act_params)
+        self._weight_fid = register_format(self.weight_params)
+        self._bias_fid = None if self.bias_params is None else register_format(self.bias_params)
+        self._round_mode_int = int(self.rounding_mode)
+        self._gemm_round_mode_int = int(self.gemm_round_mode)
+
+    def _rebuild_formats(self, rule, values):
+        derive = _FORMAT_RULES[rule]
+        for role, value in zip(_ROLES, values):
+            old = getattr(self, f"{role}_params")
+            if old is None or value is None:
+                continue
+            k, p, saturating, bias = derive(old.total_bits - old.mantissa_bits - 1,
+                                            old.mantissa_bits, old, value)
+            setattr(self, f"{role}_params", create_p3109_params(k, p, True, saturating, bias))
+        self._refresh_traced_ids()
 
     def set_mantissa(self, activ_mant, weight_mant, bias_mant):
-        act_exp = self.act_params.total_bits - self.act_params.mantissa_bits - 1
-        weight_exp = self.weight_params.total_bits - self.weight_params.mantissa_bits - 1
-        bias_exp = self.bias_params.total_bits - self.bias_params.mantissa_bits - 1
-        self.act_params = create_p3109_params(1 + act_exp + activ_mant, activ_mant + 1, True, True)
-        self.weight_params = create_p3109_params(1 + weight_exp + weight_mant, weight_mant + 1,
-                                                 True, True)
-        self.bias_params = create_p3109_params(1 + bias_exp + bias_mant, bias_mant + 1, True, True)
-        self._refresh_traced_ids()
+        self._rebuild_formats("set_mantissa", (activ_mant, weight_mant, bias_mant))
 
     def set_exponent(self, activ_exp, weight_exp, bias_exp):
-        act_mant = self.act_params.mantissa_bits
-        weight_mant = self.weight_params.mantissa_bits
-        bias_mant = self.bias_params.mantissa_bits
-        self.act_params = create_p3109_params(1 + activ_exp + act_mant, act_mant + 1, True, True)
-        self.weight_params = create_p3109_params(1 + weight_exp + weight_mant, weight_mant + 1,
-                                                 True, True)
-        self.bias_params = create_p3109_params(1 + bias_exp + bias_mant, bias_mant + 1, True, True)
-        self._refresh_traced_ids()
+        self._rebuild_formats("set_exponent", (activ_exp, weight_exp, bias_exp))
 
     def set_exponentbias(self, activ_expbias, weight_expbias, bias_expbias):
-        self.act_params = create_p3109_params(self.act_params.total_bits,
-                                              self.act_params.mantissa_bits, True, True,
-                                              activ_expbias)
-        self.weight_params = create_p3109_params(self.weight_params.total_bits,
-                                                 self.weight_params.mantissa_bits, True, True,
-                                                 weight_expbias)
-        self.bias_params = create_p3109_params(self.bias_params.total_bits,
-                                               self.bias_params.mantissa_bits, True, True,
-                                               bias_expbias)
-        self._refresh_traced_ids()
+        self._rebuild_formats("set_exponentbias", (activ_expbias, weight_expbias, bias_expbias))
 
     def set_saturation_mode(self, activ_sat, weight_sat, bias_sat):
-        self.act_params = create_p3109_params(self.act_params.total_bits,
-                                              self.act_params.mantissa_bits, True, activ_sat)
-        self.weight_params = create_p3109_params(self.weight_params.total_bits,
-                                                 self.weight_params.mantissa_bits, True,
-                                                 weight_sat)
-        self.bias_params = create_p3109_params(self.bias_params.total_bits,
-                                               self.bias_params.mantissa_bits, True, bias_sat)
-        self._refresh_traced_ids()
+        self._rebuild_formats("set_saturation_mode", (activ_sat, weight_sat, bias_sat))
 
     def set_accumulation_precision(self, accum_mant_bits):
         self.accum_mant_bits = accum_mant_bits
 
+    def accum_spec(self) -> ops.AccumSpec:
+        if self.accum_format is not None:
+            return ops.AccumSpec(fmt=self.accum_format, chunk=self.accum_chunk)
+        return ops.AccumSpec(mant_bits=int(self.accum_mant_bits), chunk=self.accum_chunk)
 
-class LoF_Linear(_FormatSetters, nn.Module):
+    def _make_parameters(self, weight_shape, n_out, bias):
+        self.weight = nn.Parameter(torch.empty(*weight_shape))
+        nn.init.kaiming_uniform_(self.weight)
+        if bias:
+            self.bias = nn.Parameter(torch.zeros(n_out))
+        else:
+            self.register_parameter("bias", None)
+
+    def _adopt(self, source: nn.Module):
+        """Take over a float layer's weight and bias (copies)."""
+        self.weight = nn.Parameter(source.weight.detach().clone())
+        if source.bias is not None:
+            self.bias = nn.Parameter(source.bias.detach().clone())
+        return self
+
+
+class LoF_Linear(_LoFLayer):
     """Drop-in for the reference LoF_Linear (layers.py:221-447) with a differentiable GEMM.
 
     Extra keyword arguments (all optional): ``accum_format`` (round the accumulator to a
     full format instead of ``accum_mant_bits`` mantissa bits), ``accum_chunk`` (k values
     accumulated in FP32 between roundings; 1 = after every FMA as in the reference) and
     ``grad_spec`` (a LinearGradSpec: format / mode / stoch_len / seed for grad_out)."""
-
-    @staticmethod
-    def _resolve_params(exp, mant, params):
-        if params is not None:
-            return params
-        if exp is not None and mant is not None:
-            return create_p3109_params(exp + mant + 1, mant)
-        raise ValueError("Must provide either (exp, mant) or params for each tensor")
 
     def __init__(self, in_features, out_features, act_params=None, weight_params=None,
                  bias=True, bias_params=None, act_exp=None, act_mant=None, weight_exp=None,
@@ -274,61 +346,36 @@ class LoF_Linear(_FormatSetters, nn.Module):
                  mx_block_size=32, scale_format=None, *, accum_format=None, accum_chunk=1,
                  grad_spec: LinearGradSpec | None = None):
         super().__init__()
-        if scaling not in ("per_tensor", "mx"):
-            raise ValueError(f"scaling must be 'per_tensor' or 'mx', got {scaling!r}")
         self.in_features = in_features
         self.out_features = out_features
-        self.rounding_mode = (rounding_mode if rounding_mode is not None
-                              else RoundingMode.RoundToNearestEven)
-        self.act_scale_factor = act_scale_factor
-        self.w_scale_factor = w_scale_factor
-        self.b_scale_factor = b_scale_factor
-        self.scaling = scaling
-        self.mx_block_size = mx_block_size
-        self.scale_format = scale_format if scale_format is not None else create_e8m0_params()
-        if scaling == "mx" and in_features % mx_block_size != 0:
-            raise ValueError(f"mx scaling needs in_features ({in_features}) divisible by "
-                             f"mx_block_size ({mx_block_size})")
-        self.accum_mant_bits = accum_mant_bits
-        self.gemm_round_mode = (gemm_round_mode if gemm_round_mode is not None
-                                else RoundingMode.RoundToNearestEven)
-        self.stochastic_rounding_bits = stochastic_rounding_bits
-        self.hadamard_transform = hadamard_transform
-        self.hadamard_block_size = hadamard_block_size
-        self.accum_format = accum_format
-        self.accum_chunk = int(accum_chunk)
+        self._init_quantisation(
+            formats=((act_params, act_exp, act_mant), (weight_params, weight_exp, weight_mant),
+                     (bias_params, bias_exp, bias_mant)),
+            rounding_mode=rounding_mode,
+            scale_factors=(act_scale_factor, w_scale_factor, b_scale_factor),
+            accum_mant_bits=accum_mant_bits, gemm_round_mode=gemm_round_mode,
+            stochastic_rounding_bits=stochastic_rounding_bits, scaling=scaling,
+            mx_block_size=mx_block_size, scale_format=scale_format,
+            hadamard=(hadamard_transform, hadamard_block_size), accum_format=accum_format,
+            accum_chunk=accum_chunk)
+        if scaling == "mx":   # blocks run along K, the last axis of x and of the weight
+            _check_blocks(in_features, mx_block_size, "in_features")
         self.grad_spec = grad_spec if grad_spec is not None else LinearGradSpec()
-        self.act_params = (act_params if act_params is not None else
-                           create_p3109_params(act_exp + act_mant + 1, act_mant + 1, True, True))
-        self.weight_params = (weight_params if weight_params is not None else
-                              create_p3109_params(weight_exp + weight_mant + 1, weight_mant + 1,
-                                                  True, True))
-        if bias_params is not None:
-            self.bias_params = bias_params
-        elif bias_exp is not None and bias_mant is not None:
-            self.bias_params = create_p3109_params(bias_exp + bias_mant + 1, bias_mant + 1, True,
-                                                   True)
-        else:
-            self.bias_params = None
-        self._refresh_traced_ids()
-        self.weight = nn.Parameter(torch.empty(out_features, in_features))
-        if bias:
-            self.bias = nn.Parameter(torch.zeros(out_features))
-        else:
-            self.register_parameter("bias", None)
-        nn.init.kaiming_uniform_(self.weight)
+        self._make_parameters((out_features, in_features), out_features, bias)
 
-    def _refresh_traced_ids(self):
-        self._act_fid = register_format(self.act_params)
-        self._weight_fid = register_format(self.weight_params)
-        self._bias_fid = register_format(self.bias_params) if self.bias_params is not None else None
-        self._round_mode_int = int(self.rounding_mode)
-        self._gemm_round_mode_int = int(self.gemm_round_mode)
-
-    def accum_spec(self) -> ops.AccumSpec:
-        if self.accum_format is not None:
-            return ops.AccumSpec(fmt=self.accum_format, chunk=self.accum_chunk)
-        return ops.AccumSpec(mant_bits=int(self.accum_mant_bits), chunk=self.accum_chunk)
+    @classmethod
+    def from_linear(cls, linear: nn.Linear, act_params, weight_params, bias_params=None,
+                    rounding_mode=None, accum_mant_bits=23, gemm_round_mode=None,
+                    stochastic_rounding_bits=0, hadamard_transform=False,
+                    hadamard_block_size=None, **kw):
+        """A LoF_Linear holding copies of ``linear``'s weight and bias (layers.py:301-323)."""
+        return cls(linear.in_features, linear.out_features, act_params=act_params,
+                   weight_params=weight_params, bias=linear.bias is not None,
+                   bias_params=bias_params, rounding_mode=rounding_mode,
+                   accum_mant_bits=accum_mant_bits, gemm_round_mode=gemm_round_mode,
+                   stochastic_rounding_bits=stochastic_rounding_bits,
+                   hadamard_transform=hadamard_transform,
+                   hadamard_block_size=hadamard_block_size, **kw)._adopt(linear)
 
     def quantized_operands(self, x):
         """(x_q, w_q, alpha) exactly as forward() feeds them to the GEMM."""
@@ -366,7 +413,23 @@ class LoF_Linear(_FormatSetters, nn.Module):
                 f"accum_chunk={self.accum_chunk}")
 
 
-class LoF_Conv2d(_FormatSetters, nn.Module):
+def _as_pair(v):
+    return tuple(v) if isinstance(v, (tuple, list)) else (v, v)
+
+
+def _hadamard_block(block_size, channels):
+    """The Walsh-Hadamard block along the input channels of one group: ``block_size`` or,
+    when None, all of them; a power of two dividing the channel count."""
+    b = channels if block_size is None else block_size
+    if b <= 0 or b & (b - 1):
+        raise ValueError(f"the Hadamard block ({b}) must be a power of two")
+    if channels % b:
+        raise ValueError(f"the Hadamard block ({b}) must divide the {channels} input channels "
+                         "of a group")
+    return b
+
+
+class LoF_Conv2d(_LoFLayer):
     """Drop-in for the reference LoF_Conv2d (layers.py:450-800): per-tensor or MX
     quantisation of the activation and weight, then the emulated-accumulation convolution,
     then the (quantised) bias.  Forward only, as in the reference.
@@ -381,13 +444,7 @@ class LoF_Conv2d(_FormatSetters, nn.Module):
     ``groups > 1`` runs one convolution per group.  Extra keywords: ``accum_format``,
     ``accum_chunk`` (as in LoF_Linear)."""
 
-    @staticmethod
-    def _resolve_params(exp, mant, params):
-        if params is not None:
-            return params
-        if exp is not None and mant is not None:
-            return create_p3109_params(exp + mant + 1, mant + 1)
-        raise ValueError("Must provide either (exp, mant) or params for each tensor")
+    _BACKENDS = ("im2col", "cutlass")
 
     def __init__(self, in_channels, out_channels, kernel_size, stride=1, padding=0,
                  dilation=1, groups=1, act_params=None, weight_params=None, bias=True,
@@ -399,110 +456,56 @@ class LoF_Conv2d(_FormatSetters, nn.Module):
                  mx_block_size=32, scale_format=None, backend="im2col", *, accum_format=None,
                  accum_chunk=1):
         super().__init__()
-        if scaling not in ("per_tensor", "mx"):
-            raise ValueError(f"scaling must be 'per_tensor' or 'mx', got {scaling!r}")
-        if backend not in ("im2col", "cutlass"):
-            raise ValueError(f"backend must be 'im2col' or 'cutlass', got {backend!r}")
-        if backend == "cutlass":
-            if scaling == "mx":
-                raise NotImplementedError("backend='cutlass' does not yet support scaling='mx'")
-            if groups != 1:
-                raise NotImplementedError("backend='cutlass' currently supports groups=1 only")
-        pair = lambda v: v if isinstance(v, tuple) else (v, v)   # noqa: E731
+        if backend not in self._BACKENDS:
+            raise ValueError(f"unknown backend {backend!r}: use one of {self._BACKENDS}")
         self.backend = backend
-        self.in_channels = in_channels
-        self.out_channels = out_channels
-        self.kernel_size = pair(kernel_size)
-        self.stride = pair(stride)
-        self.padding = pair(padding)
-        self.dilation = pair(dilation)
-        self.groups = groups
-        self.rounding_mode = (rounding_mode if rounding_mode is not None
-                              else RoundingMode.RoundToNearestEven)
-        self.act_scale_factor = act_scale_factor
-        self.w_scale_factor = w_scale_factor
-        self.b_scale_factor = b_scale_factor
-        self.accum_mant_bits = accum_mant_bits
-        self.gemm_round_mode = (gemm_round_mode if gemm_round_mode is not None
-                                else RoundingMode.RoundToNearestEven)
-        self.stochastic_rounding_bits = stochastic_rounding_bits
-        self.accum_format = accum_format
-        self.accum_chunk = int(accum_chunk)
-        self.scaling = scaling
-        self.mx_block_size = mx_block_size
-        self.scale_format = scale_format if scale_format is not None else create_e8m0_params()
+        self.in_channels, self.out_channels, self.groups = in_channels, out_channels, groups
+        self.kernel_size = _as_pair(kernel_size)
+        self.stride = _as_pair(stride)
+        self.padding = _as_pair(padding)
+        self.dilation = _as_pair(dilation)
+        self._init_quantisation(
+            formats=((act_params, act_exp, act_mant), (weight_params, weight_exp, weight_mant),
+                     (bias_params, bias_exp, bias_mant)),
+            rounding_mode=rounding_mode,
+            scale_factors=(act_scale_factor, w_scale_factor, b_scale_factor),
+            accum_mant_bits=accum_mant_bits, gemm_round_mode=gemm_round_mode,
+            stochastic_rounding_bits=stochastic_rounding_bits, scaling=scaling,
+            mx_block_size=mx_block_size, scale_format=scale_format,
+            hadamard=(hadamard_transform, hadamard_block_size), accum_format=accum_format,
+            accum_chunk=accum_chunk)
+        # what each path supports: the direct convolution takes one group and per-tensor
+        # scales; MX blocks run along the unfolded reduction axis C/groups * kH * kW
+        if (backend == "cutlass" or scaling == "mx") and groups != 1:
+            raise NotImplementedError(f"{backend} / {scaling}: grouped convolution is not "
+                                      "supported on this path (groups must be 1)")
+        if backend == "cutlass" and scaling == "mx":
+            raise NotImplementedError("the cutlass backend takes per-tensor scaling only")
         if scaling == "mx":
-            kH, kW = self.kernel_size
-            K = (in_channels // groups) * kH * kW
-            if groups != 1:
-                raise NotImplementedError(
-                    "scaling='mx' for LoF_Conv2d currently supports groups=1 only")
-            if K % mx_block_size != 0:
-                raise ValueError(
-                    f"mx scaling needs the im2col K=(C_in/groups)*kH*kW={K} divisible by "
-                    f"mx_block_size ({mx_block_size})")
-        self.hadamard_transform = hadamard_transform
-        self.hadamard_block_size = hadamard_block_size
+            kh, kw = self.kernel_size
+            _check_blocks(in_channels // groups * kh * kw, mx_block_size,
+                          "the unfolded reduction axis (C/groups * kH * kW)")
         if hadamard_transform:
-            c_per_group_in = in_channels // groups
-            eff = hadamard_block_size if hadamard_block_size is not None else c_per_group_in
-            if eff <= 0 or (eff & (eff - 1)) != 0:
-                raise ValueError(
-                    "hadamard_transform=True requires (hadamard_block_size or "
-                    f"in_channels//groups) to be a power of 2, got {eff}")
-            if c_per_group_in % eff != 0:
-                raise ValueError(f"hadamard_block_size={eff} must divide in_channels//groups="
-                                 f"{c_per_group_in}")
-        self.act_params = (act_params if act_params is not None else
-                           create_p3109_params(act_exp + act_mant + 1, act_mant + 1, True, True))
-        self.weight_params = (weight_params if weight_params is not None else
-                              create_p3109_params(weight_exp + weight_mant + 1, weight_mant + 1,
-                                                  True, True))
-        if bias_params is not None:
-            self.bias_params = bias_params
-        elif bias_exp is not None and bias_mant is not None:
-            self.bias_params = create_p3109_params(bias_exp + bias_mant + 1, bias_mant + 1, True,
-                                                   True)
-        else:
-            self.bias_params = None
-        self._refresh_traced_ids()
-        self.weight = nn.Parameter(torch.empty(out_channels, in_channels // groups,
-                                               *self.kernel_size))
-        if bias:
-            self.bias = nn.Parameter(torch.zeros(out_channels))
-        else:
-            self.register_parameter("bias", None)
-        nn.init.kaiming_uniform_(self.weight)
+            _hadamard_block(hadamard_block_size, in_channels // groups)
+        self._make_parameters((out_channels, in_channels // groups, *self.kernel_size),
+                              out_channels, bias)
 
     @classmethod
     def from_conv2d(cls, conv: nn.Conv2d, act_params, weight_params, bias_params=None,
                     rounding_mode=None, accum_mant_bits=23, gemm_round_mode=None,
                     stochastic_rounding_bits=0, hadamard_transform=False,
-                    hadamard_block_size=None):
-        layer = cls(conv.in_channels, conv.out_channels, conv.kernel_size, stride=conv.stride,
-                    padding=conv.padding, dilation=conv.dilation, groups=conv.groups,
-                    act_params=act_params, weight_params=weight_params,
-                    bias=conv.bias is not None, bias_params=bias_params,
-                    rounding_mode=rounding_mode, accum_mant_bits=accum_mant_bits,
-                    gemm_round_mode=gemm_round_mode,
-                    stochastic_rounding_bits=stochastic_rounding_bits,
-                    hadamard_transform=hadamard_transform,
-                    hadamard_block_size=hadamard_block_size)
-        layer.weight = nn.Parameter(conv.weight.detach().clone())
-        if conv.bias is not None:
-            layer.bias = nn.Parameter(conv.bias.detach().clone())
-        return layer
-
-    def _refresh_traced_ids(self):
-        self._act_fid = register_format(self.act_params)
-        self._weight_fid = register_format(self.weight_params)
-        self._bias_fid = register_format(self.bias_params) if self.bias_params is not None else None
-        self._round_mode_int = int(self.rounding_mode)
-
-    def accum_spec(self) -> ops.AccumSpec:
-        if self.accum_format is not None:
-            return ops.AccumSpec(fmt=self.accum_format, chunk=self.accum_chunk)
-        return ops.AccumSpec(mant_bits=int(self.accum_mant_bits), chunk=self.accum_chunk)
+                    hadamard_block_size=None, **kw):
+        """A LoF_Conv2d with ``conv``'s geometry holding copies of its weight and bias
+        (layers.py:560-587)."""
+        return cls(conv.in_channels, conv.out_channels, conv.kernel_size, stride=conv.stride,
+                   padding=conv.padding, dilation=conv.dilation, groups=conv.groups,
+                   act_params=act_params, weight_params=weight_params,
+                   bias=conv.bias is not None, bias_params=bias_params,
+                   rounding_mode=rounding_mode, accum_mant_bits=accum_mant_bits,
+                   gemm_round_mode=gemm_round_mode,
+                   stochastic_rounding_bits=stochastic_rounding_bits,
+                   hadamard_transform=hadamard_transform,
+                   hadamard_block_size=hadamard_block_size, **kw)._adopt(conv)
 
     def _bias(self, out, scaled=True):
         if self.bias is None:
@@ -603,6 +606,34 @@ class _PWLSiLUApprox(torch.autograd.Function):
         return grad_output * (s * (1 + x * (1 - s))), None, None
 
 
+def _silu(x):
+    return x * torch.sigmoid(x)
+
+
+# The table's knot values under each lut_method (layers.py:1098-1121), from the knots t_0..t_N
+# and the segment midpoints c_0..c_{N-1}.  These formulas define the table the kernel reads.
+def _knots_uniform(t, c):
+    return _silu(t)
+
+
+def _knots_midpoint(t, c):
+    v = _silu(t)
+    v[1:-1] = _silu(c[:-1])   # interior knot j takes SiLU at the midpoint of segment j - 1
+    return v
+
+
+def _knots_minimax(t, c):
+    v = _silu(t)
+    gap = _silu(c) - 0.5 * (v[:-1] + v[1:])   # SiLU above the chord at each midpoint
+    # a knot rises by half the mean gap of the segments that meet at it (one at the ends)
+    edge = gap[[0, -1]] / 2
+    return v + torch.cat((edge[:1], (gap[:-1] + gap[1:]) / 4, edge[1:]))
+
+
+_LUT_RULES = {"uniform": _knots_uniform, "midpoint": _knots_midpoint,
+              "minimax": _knots_minimax}
+
+
 class PWLSiLU(nn.Module):
     """SiLU as a piecewise-linear table of 2^lut_bits uniform segments on [-R, R], the
     asymptotes 0 / x outside (layers.py:1072-1137).  The table's knot values follow
@@ -611,42 +642,27 @@ class PWLSiLU(nn.Module):
     the adjacent segments' midpoint errors, halving the maximum error of the chord).
     ``use_ext=False`` runs the torch op chain the kernel replaces."""
 
+    _MAX_SEGMENTS = 4096   # the largest table the kernel stages in LDS
+
     def __init__(self, R=8.0, lut_bits=4, lut_method="minimax", use_ext=True):
         super().__init__()
         self.R = float(R)
         self.lut_bits = lut_bits
         self.lut_size = 1 << lut_bits
-        if self.lut_size % 2 != 0:
-            raise ValueError("lut_size must be even so x=0 is a knot.")
-        self.use_ext = use_ext and self.lut_size <= 4096
+        if self.lut_size < 2:
+            raise ValueError("PWLSiLU needs an even number of segments (lut_bits >= 1), so "
+                             "that x = 0 is a knot")
+        self.use_ext = use_ext and self.lut_size <= self._MAX_SEGMENTS
         self.register_buffer("lut", torch.empty(self.lut_size + 1))
         self.init_lut(lut_method)
 
-    @staticmethod
-    def _silu(x):
-        return x * torch.sigmoid(x)
-
     def init_lut(self, method="minimax"):
-        N = self.lut_size
-        knots = torch.linspace(-self.R, self.R, N + 1, dtype=self.lut.dtype)
-        at_knots = self._silu(knots)
-        mids = 0.5 * (knots[:-1] + knots[1:])
-        if method == "uniform":
-            lut = at_knots.clone()
-        elif method == "midpoint":
-            lut = at_knots.clone()
-            lut[1:-1] = self._silu(mids)[:-1]
-        elif method == "minimax":
-            err = self._silu(mids) - 0.5 * (at_knots[:-1] + at_knots[1:])   # chord error
-            shift = torch.empty_like(at_knots)
-            shift[0] = err[0] / 2
-            shift[1:-1] = (err[:-1] + err[1:]) / 4
-            shift[-1] = err[-1] / 2
-            lut = at_knots + shift
-        else:
-            raise ValueError(f"Unknown lut_method {method!r}")
+        rule = _LUT_RULES.get(method)
+        if rule is None:
+            raise ValueError(f"lut_method must be one of {sorted(_LUT_RULES)} (got {method!r})")
+        t = torch.linspace(-self.R, self.R, self.lut_size + 1, dtype=self.lut.dtype)
         with torch.no_grad():
-            self.lut.copy_(lut)
+            self.lut.copy_(rule(t, 0.5 * (t[:-1] + t[1:])))
         self.lut_method = method
 
     def forward(self, x):
@@ -655,12 +671,14 @@ class PWLSiLU(nn.Module):
         return self._forward_ref(x)
 
     def _forward_ref(self, x):
-        N, R = self.lut_size, self.R
-        u = (x.clamp(-R, R) + R) * (N / (2 * R))
-        idx = u.long().clamp_(max=N - 1)
-        frac = u - idx.to(x.dtype)
-        out = torch.lerp(self.lut[idx], self.lut[idx + 1], frac)
-        return torch.where(x.abs() > R, torch.relu(x), out)
+        """The torch op chain the kernel replaces (layers.py:1131-1150): the clamped input's
+        position in knot units, its segment, linear interpolation between the segment's two
+        knots, and relu(x) beyond [-R, R]."""
+        R, n = self.R, self.lut_size
+        pos = (x.clamp(-R, R) + R) * (n / (2 * R))
+        seg = pos.long().clamp_(max=n - 1)
+        inside = torch.lerp(self.lut[seg], self.lut[seg + 1], pos - seg.to(x.dtype))
+        return torch.where(x.abs() > R, torch.relu(x), inside)
 
 
 def _not_in_scope(name):
@@ -680,5 +698,12 @@ FISRBatchNorm = _not_in_scope("FISRBatchNorm")
 PWLBatchNorm = _not_in_scope("PWLBatchNorm")
 
 
-def replace_mha_with_explicit(*args, **kwargs):
-    raise NotImplementedError("replace_mha_with_explicit: outside the accelerated hot path")
+def replace_mha_with_explicit(model: nn.Module) -> nn.Module:
+    """The reference swaps every nn.MultiheadAttention for its explicit attention layer
+    (layers.py:1499-1510), which this package does not accelerate: a model without one is
+    returned unchanged, a model with one raises NotImplementedError."""
+    for name, m in model.named_modules():
+        if isinstance(m, nn.MultiheadAttention):
+            raise NotImplementedError(f"replace_mha_with_explicit: '{name}' is an "
+                                      "nn.MultiheadAttention, outside the accelerated hot path")
+    return model
