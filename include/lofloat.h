@@ -291,6 +291,20 @@ LOF_API lof_status lof_grouped_autotune(const lof_group_item* items, int32_t cou
                                         lof_stream_t stream, int32_t* best_config,
                                         float* best_ms);
 
+/* The grouped launch of lof_grouped_create + lof_grouped_run in one call that allocates
+ * nothing.  No counterpart in the reference: it is for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  The work queue, the job
+ * start table and the task records live in one fixed device workspace of the library
+ * (about 170 KB per GPU, held until the process ends), so count <= 256 items
+ * (LOF_ERR_UNSUPPORTED beyond).  The task table is copied from host memory into the
+ * workspace on `stream` (a pageable copy, which HIP performs synchronously), then one
+ * kernel runs it; outputs are bit-identical to lof_grouped_run on the same items.  Every
+ * call on a GPU shares that GPU's workspace: calls that overlap on one GPU (other streams,
+ * other threads' streams) race and must be ordered by the caller.  config as in
+ * lof_grouped_create. */
+LOF_API lof_status lof_grouped_run_noalloc(const lof_group_item* items, int32_t count,
+                                           int32_t config, lof_stream_t stream);
+
 /* HIP graph helpers: capture plan runs into a graph and replay it. */
 typedef struct lof_graph lof_graph;
 LOF_API lof_status lof_graph_capture_grouped(lof_grouped* plan, int32_t separate,

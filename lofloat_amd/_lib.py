@@ -103,6 +103,7 @@ SIGNATURES = {
     "lof_cast_to_rep_f32": (_i32, [_vp, _vp, _i64, _P(PackFormat), RoundSpec, _vp, _u64, _vp]),
     "lof_cast_from_rep_f32": (_i32, [_vp, _vp, _i64, _P(PackFormat), _vp]),
     "lof_grouped_run": (_i32, [_vp, _vp]),
+    "lof_grouped_run_noalloc": (_i32, [_P(GroupItem), _i32, _i32, _vp]),
     "lof_grouped_run_separate": (_i32, [_vp, _vp]),
     "lof_grouped_destroy": (_i32, [_vp]),
     "lof_grouped_num_configs": (_i32, []),

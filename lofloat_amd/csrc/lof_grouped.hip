@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -214,9 +215,26 @@ constexpr int kNumGConfigs = sizeof(kGConfigs) / sizeof(kGConfigs[0]);
 int g_default_gconfig = 4;   // 64 x 64 VALU, 128 x 64 MFMA, 4 rounding sub-tiles
 std::mutex g_gmu;
 
+// Fixed device workspace of lof_grouped_run_noalloc: the work-queue counters, the job
+// start table and the task table, laid out as
+//   [0, 256)                      two unsigned long long counters (then padding)
+//   [kNoallocStartsOff, ...)      kNoallocMaxItems int64_t tile starts
+//   [kNoallocTasksOff, ...)       kNoallocMaxItems GroupTask records
+constexpr int kNoallocMaxItems = 256;
+constexpr size_t kNoallocStartsOff = 256;
+constexpr size_t kNoallocTasksOff =
+    kNoallocStartsOff + ((kNoallocMaxItems * sizeof(int64_t) + 255) / 256) * 256;
+constexpr size_t kNoallocBytes = kNoallocTasksOff + kNoallocMaxItems * sizeof(GroupTask);
+static_assert(alignof(GroupTask) <= 256, "task table alignment");
+
 }  // namespace
 
 }  // namespace lofamd
+
+// Device workspace of lof_grouped_run_noalloc (kNoallocBytes, about 170 KB per GPU, held
+// until the process ends).  Host code never uses this name as an address: it takes each
+// device's address from hipGetSymbolAddress once and passes that.
+static __device__ __align__(256) unsigned char scratch[lofamd::kNoallocBytes];
 
 using namespace lofamd;
 
@@ -295,6 +313,73 @@ lof_status free_plan(lof_grouped* p) {
     return LOF_OK;
 }
 
+
+int resolve_gconfig(int config) {
+    if (config >= 0) return config;
+    std::lock_guard<std::mutex> lk(g_gmu);
+    return g_default_gconfig;
+}
+
+// Validates every item and lays the jobs out in queue order: GEMMs first, largest first,
+// then rounding jobs, largest first; jobs with no tiles are dropped.  work / tiles / order
+// are caller-provided scratch of `count` entries; tasks / starts receive *ntasks entries.
+lof_status order_tasks(const lof_group_item* items, int count, const GroupedConfig& cfg,
+                       GroupTask* work, int64_t* tiles, int* order, GroupTask* tasks,
+                       int64_t* starts, int* ntasks, int64_t* total) {
+    for (int i = 0; i < count; ++i) {
+        lof_status s = build_task(items[i], cfg, &work[i], &tiles[i]);
+        if (s != LOF_OK) return s;
+        order[i] = i;
+    }
+    std::stable_sort(order, order + count, [&](int a, int b) {
+        return task_cost(work[a], tiles[a]) > task_cost(work[b], tiles[b]);
+    });
+    int nt = 0;
+    int64_t tot = 0;
+    for (int i = 0; i < count; ++i) {
+        const int idx = order[i];
+        if (tiles[idx] == 0) continue;
+        starts[nt] = tot;
+        tasks[nt] = work[idx];
+        tot += tiles[idx];
+        ++nt;
+    }
+    if (tot >= (int64_t)1 << 30)
+        return set_error(LOF_ERR_UNSUPPORTED, "lof_grouped: more than 2^30 tiles");
+    *ntasks = nt;
+    *total = tot;
+    return LOF_OK;
+}
+
+// Persistent grid: every workgroup the device holds at once, at most one per tile.
+lof_status persistent_grid(const GroupedConfig& cfg, int device, int64_t total, int* grid) {
+    int cus = 0, per_cu = 0;
+    lof_status s = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                                   device), "device attribute");
+    if (s == LOF_OK)
+        s = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &per_cu, reinterpret_cast<const void*>(cfg.kernel), 256, 0),
+                      "occupancy");
+    if (s == LOF_OK) {
+        const int64_t resident = (int64_t)cus * std::max(per_cu, 1);
+        *grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, total));
+    }
+    return s;
+}
+
+// Host state of lof_grouped_run_noalloc: each device's workspace address and one host image
+// of the workspace with the planner's per-item scratch, all fixed-size.
+constexpr int kNoallocMaxDevices = 64;
+struct NoallocImage {
+    alignas(256) unsigned char bytes[kNoallocBytes];
+    GroupTask work[kNoallocMaxItems];
+    int64_t tiles[kNoallocMaxItems];
+    int order[kNoallocMaxItems];
+};
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[kNoallocMaxDevices] = {};
+NoallocImage g_noalloc_image;
+
 }  // namespace
 
 extern "C" {
@@ -315,49 +400,23 @@ LOF_API lof_status lof_grouped_create(const lof_group_item* items, int32_t count
     *plan = nullptr;
     if (config >= kNumGConfigs)
         return set_error(LOF_ERR_INVALID_ARGUMENT, "lof_grouped_create: config out of range");
-    if (config < 0) {
-        std::lock_guard<std::mutex> lk(g_gmu);
-        config = g_default_gconfig;
-    }
+    config = resolve_gconfig(config);
     const GroupedConfig& cfg = kGConfigs[config];
-    std::vector<GroupTask> tasks((size_t)count);
-    std::vector<int64_t> tiles((size_t)count);
-    for (int i = 0; i < count; ++i) {
-        lof_status s = build_task(items[i], cfg, &tasks[i], &tiles[i]);
-        if (s != LOF_OK) return s;
-    }
-    std::vector<int> order((size_t)count);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return task_cost(tasks[a], tiles[a]) > task_cost(tasks[b], tiles[b]);
-    });
     auto* p = new lof_grouped();
     p->config = config;
-    std::vector<int64_t> starts;
-    for (int idx : order) {
-        if (tiles[idx] == 0) continue;
-        starts.push_back(p->total);
-        p->tasks.push_back(tasks[idx]);
-        p->total += tiles[idx];
-    }
-    p->ntasks = (int)p->tasks.size();
-    if (p->total >= (int64_t)1 << 30) {
-        free_plan(p);
-        return set_error(LOF_ERR_UNSUPPORTED, "lof_grouped_create: more than 2^30 tiles");
-    }
-    lof_status s = check_hip(hipGetDevice(&p->device), "hipGetDevice");
-    int cus = 0, per_cu = 0;
-    if (s == LOF_OK)
-        s = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
-                                            p->device), "device attribute");
-    if (s == LOF_OK)
-        s = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                          &per_cu, reinterpret_cast<const void*>(cfg.kernel), 256, 0),
-                      "occupancy");
+    p->tasks.resize((size_t)count);
+    std::vector<int64_t> starts((size_t)count);
+    std::vector<GroupTask> work((size_t)count);
+    std::vector<int64_t> tiles((size_t)count);
+    std::vector<int> order((size_t)count);
+    lof_status s = order_tasks(items, count, cfg, work.data(), tiles.data(), order.data(),
+                               p->tasks.data(), starts.data(), &p->ntasks, &p->total);
     if (s == LOF_OK) {
-        const int64_t resident = (int64_t)cus * std::max(per_cu, 1);
-        p->grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, p->total));
+        p->tasks.resize((size_t)p->ntasks);
+        starts.resize((size_t)p->ntasks);
+        s = check_hip(hipGetDevice(&p->device), "hipGetDevice");
     }
+    if (s == LOF_OK) s = persistent_grid(cfg, p->device, p->total, &p->grid);
     if (s == LOF_OK && p->ntasks > 0) {
         s = check_hip(hipMalloc(&p->d_tasks, sizeof(GroupTask) * p->tasks.size()), "hipMalloc");
         if (s == LOF_OK)
@@ -465,6 +524,58 @@ LOF_API lof_status lof_grouped_autotune(const lof_group_item* items, int32_t cou
     if (best_config) *best_config = best;
     if (best_ms) *best_ms = best_t;
     return LOF_OK;
+}
+
+LOF_API lof_status lof_grouped_run_noalloc(const lof_group_item* items, int32_t count,
+                                           int32_t config, lof_stream_t stream) {
+    if (count < 0 || (count > 0 && !items))
+        return set_error(LOF_ERR_INVALID_ARGUMENT, "lof_grouped_run_noalloc: bad arguments");
+    if (count > kNoallocMaxItems)
+        return set_error(LOF_ERR_UNSUPPORTED,
+                         "lof_grouped_run_noalloc: %d items exceed the fixed workspace's %d "
+                         "(use lof_grouped_create for larger lists)", count, kNoallocMaxItems);
+    if (config >= kNumGConfigs)
+        return set_error(LOF_ERR_INVALID_ARGUMENT, "lof_grouped_run_noalloc: config out of range");
+    config = resolve_gconfig(config);
+    const GroupedConfig& cfg = kGConfigs[config];
+    int device = 0;
+    lof_status s = check_hip(hipGetDevice(&device), "hipGetDevice");
+    if (s != LOF_OK) return s;
+    if (device < 0 || device >= kNoallocMaxDevices)
+        return set_error(LOF_ERR_UNSUPPORTED, "lof_grouped_run_noalloc: device %d", device);
+    // One host image of the workspace, built and copied under the lock: the copy from
+    // pageable host memory has consumed the image when hipMemcpyAsync returns.
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    unsigned char* dptr = g_noalloc_ptr[device];
+    if (!dptr) {   // first call on this device (which is current): its copy's address
+        void* ptr = nullptr;
+        s = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+        if (s != LOF_OK) return s;
+        dptr = g_noalloc_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    NoallocImage& im = g_noalloc_image;
+    int ntasks = 0;
+    int64_t total = 0;
+    s = order_tasks(items, count, cfg, im.work, im.tiles, im.order,
+                    reinterpret_cast<GroupTask*>(im.bytes + kNoallocTasksOff),
+                    reinterpret_cast<int64_t*>(im.bytes + kNoallocStartsOff), &ntasks, &total);
+    if (s != LOF_OK) return s;
+    if (total == 0) return LOF_OK;
+    int grid = 0;
+    s = persistent_grid(cfg, device, total, &grid);
+    if (s != LOF_OK) return s;
+    // counters start at zero (the kernel's last workgroup also leaves them at zero)
+    std::memset(im.bytes, 0, kNoallocStartsOff);
+    const size_t used = kNoallocTasksOff + (size_t)ntasks * sizeof(GroupTask);
+    hipStream_t st = (hipStream_t)stream;
+    s = check_hip(hipMemcpyAsync(dptr, im.bytes, used, hipMemcpyHostToDevice, st),
+                  "lof_grouped_run_noalloc upload");
+    if (s != LOF_OK) return s;
+    hipLaunchKernelGGL(cfg.kernel, dim3((unsigned)grid), dim3(256), 0, st,
+                       reinterpret_cast<const GroupTask*>(dptr + kNoallocTasksOff),
+                       reinterpret_cast<const int64_t*>(dptr + kNoallocStartsOff), ntasks, total,
+                       reinterpret_cast<unsigned long long*>(dptr));
+    return check_hip(hipGetLastError(), "lof_grouped_run_noalloc launch");
 }
 
 LOF_API lof_status lof_graph_capture_grouped(lof_grouped* p, int32_t separate,

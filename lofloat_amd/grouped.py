@@ -236,3 +236,17 @@ def autotune(jobs, iters: int = 3) -> tuple[int, float]:
                                              ctypes.byref(best), ctypes.byref(ms)),
                "lof_grouped_autotune")
     return best.value, ms.value
+
+
+def run_noalloc(jobs, config: int = -1) -> None:
+    """Run the jobs as one grouped launch without allocating device memory
+    (lof_grouped_run_noalloc): the work list goes into the library's fixed device
+    workspace, so at most 256 jobs; outputs must already exist (``job.out``, allocated here
+    when missing, as the other entry points do).  Calls that overlap on one GPU share that
+    workspace and must be ordered by the caller."""
+    jobs = list(jobs)
+    arr = (_lib.GroupItem * max(len(jobs), 1))()
+    for i, j in enumerate(jobs):
+        arr[i] = j.item()
+    _lib.check(_lib.lib.lof_grouped_run_noalloc(arr, len(jobs), int(config), _stream()),
+               "lof_grouped_run_noalloc")

@@ -56,6 +56,12 @@ def test_validation_without_device():
     st = L.lof_virtual_round_mant_f32(None, None, 0, 24, _lib.RoundSpec(0, 0, 0), 1.0, None, 0,
                                       None)
     assert st == 2
+    # the allocation-free grouped launch has a fixed workspace of 256 items
+    items = (_lib.GroupItem * 257)()
+    st = L.lof_grouped_run_noalloc(items, 257, -1, None)
+    assert st == 2 and b"256" in L.lof_last_error()
+    st = L.lof_grouped_run_noalloc(None, 3, -1, None)
+    assert st == 1
 
 
 def test_format_descriptor_contract():
