@@ -84,8 +84,8 @@ def main():
                          "'conv' = the implicit-GEMM Conv2d; 'round' = elementwise virtual_round "
                          "of 2^24 / 2^28 values (with --no-cpu: no oracle check)")
     ap.add_argument("--chunk", type=int, default=32)
-    ap.add_argument("--accum", choices=["mant10", "5e10m"], default="5e10m")
-    ap.add_argument("--inputs", choices=["e4m3", "8e7m"], default="e4m3")
+    ap.add_argument("--accum", choices=["mant10", "mant23", "5e10m"], default="5e10m")
+    ap.add_argument("--inputs", choices=["e4m3", "8e7m", "fp32u"], default="e4m3")
     ap.add_argument("--config", type=int, default=-1, help="GEMM tile config (-1: default)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -212,8 +212,12 @@ def c1_device_values(torch, dev, n, seed, specials=0.01):
 
 
 def gemm_inputs(lof, torch, dev, inputs, n=4096):
-    fmt = lof.make_format(8, 3, 7) if inputs == "e4m3" else lof.make_format(16, 7, 127)
+    """n x n operands: "e4m3" / "8e7m" = N(0, 1/64) pre-rounded to (8,3,7) / (16,7,127);
+    "fp32u" = FP32 uniform(-1, 1), the reference harness's lof_gemm inputs."""
     g = torch.Generator(device=dev).manual_seed(3)
+    if inputs == "fp32u":
+        return tuple(torch.rand(n, n, device=dev, generator=g) * 2 - 1 for _ in range(2))
+    fmt = lof.make_format(8, 3, 7) if inputs == "e4m3" else lof.make_format(16, 7, 127)
     A = lof.virtual_round(torch.randn(n, n, device=dev, generator=g) / 64, fmt)
     B = lof.virtual_round(torch.randn(n, n, device=dev, generator=g) / 64, fmt)
     return A, B
@@ -224,8 +228,9 @@ def gemm_only(lof, torch, dev, args):
     8e7m, accumulator 10-bit mantissa or 5e10m (16,10,15), chunk --chunk."""
     n = 4096
     A, B = gemm_inputs(lof, torch, dev, args.inputs, n)
-    spec = (lof.AccumSpec(mant_bits=10, chunk=args.chunk) if args.accum == "mant10"
-            else lof.AccumSpec(fmt=lof.make_format(16, 10, 15), chunk=args.chunk))
+    spec = (lof.AccumSpec(fmt=lof.make_format(16, 10, 15), chunk=args.chunk)
+            if args.accum == "5e10m" else
+            lof.AccumSpec(mant_bits=int(args.accum[4:]), chunk=args.chunk))
     ms = timed(lambda i: lof.gemm(A, B, spec, config=args.config), args.steps, args.warmup)
     tf = 2.0 * n ** 3 / ms / 1e9
     print(json.dumps({"workload": f"gemm {n}^3 {args.inputs} inputs, {args.accum} accumulator, "
@@ -253,37 +258,46 @@ def extras(lof, torch, np, oracle, dev):
     log(f"mx variants: {out['mx_e2m1_GBs']} / {out['mx_e4m3_srA8_bits_GBs']} GB/s")
     del x, y, bits
 
-    # C3: emulated GEMM 4096^3 on e4m3 / 8e7m pre-rounded inputs.  Parity: 4096 sampled
-    # outputs (full K) against the oracle, bit for bit; the oracle's time on them gives the
-    # CPU rate in emulated multiply-adds per second.
+    # C3: emulated GEMM 4096^3 on e4m3 / 8e7m pre-rounded inputs, and the reference
+    # harness's FP32 uniform(-1, 1) operands with its 10-bit accumulator at chunk 1 (and the
+    # LoF layers' default 23-bit one).  The tile config is autotuned on each workload before
+    # it is timed.  Parity: 4096 sampled outputs (full K) against the oracle, bit for bit;
+    # the oracle's time on them gives the CPU rate in emulated multiply-adds per second.
     N = 4096
     flop = 2.0 * N ** 3
     samples = np.random.default_rng(11).choice(N * N, 4096, replace=False)
     sample_t = torch.from_numpy(samples).to(dev)
-    for inp in ("e4m3", "8e7m"):
-        A, B = gemm_inputs(lof, torch, dev, inp, N)
-        Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
-        for name, acc, oacc in (
-                ("mant10", lambda c: lof.AccumSpec(mant_bits=10, chunk=c),
-                 lambda c: oracle.accum_mant(10, c)),
-                ("5e10m", lambda c: lof.AccumSpec(fmt=fp16, chunk=c),
-                 lambda c: oracle.accum_fmt(fp16.as_tuple(), c))):
-            for chunk in (1, 8, 32):
-                spec = acc(chunk)
-                it = 2 if chunk == 1 else 4
-                ms = timed(lambda i: lof.gemm(A, B, spec), it)
-                key = f"gemm4096_{inp}_{name}_c{chunk}"
-                out[key + "_TFLOPs"] = round(flop / ms / 1e9, 3)
-                got = lof.gemm(A, B, spec).flatten()[sample_t].cpu().numpy()
-                t0 = time.perf_counter()
-                ref = oracle.gemm(Ah, Bh, oacc(chunk), samples=samples)
-                cpu_s = time.perf_counter() - t0
-                same = bool(np.array_equal(got.view(np.uint32), np.asarray(ref, np.float32).ravel().view(np.uint32)))
-                out[key + "_parity_4096_samples"] = same
-                out[key + "_cpu_oracle_GMACs"] = round(samples.size * N / cpu_s / 1e9, 4)
-                log(f"{key}: {ms:.2f} ms, {out[key + '_TFLOPs']} TFLOP/s, sampled parity {same}, "
-                    f"oracle {out[key + '_cpu_oracle_GMACs']} G emulated MAC/s")
-        del A, B, Ah, Bh
+    mant = lambda m: (lambda c: lof.AccumSpec(mant_bits=m, chunk=c),   # noqa: E731
+                      lambda c: oracle.accum_mant(m, c))
+    accs = {"mant10": mant(10), "mant23": mant(23),
+            "5e10m": (lambda c: lof.AccumSpec(fmt=fp16, chunk=c),
+                      lambda c: oracle.accum_fmt(fp16.as_tuple(), c))}
+    plan = [(inp, name, chunk) for inp in ("e4m3", "8e7m") for name in ("mant10", "5e10m")
+            for chunk in (1, 8, 32)] + [("fp32u", "mant10", 1), ("fp32u", "mant23", 1)]
+    cur = None
+    for inp, name, chunk in plan:
+        if cur != inp:
+            A, B = gemm_inputs(lof, torch, dev, inp, N)
+            Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+            cur = inp
+        acc, oacc = accs[name]
+        spec = acc(chunk)
+        cfg, _ = lof.gemm_autotune(A, B, spec, iters=1)
+        it = 2 if chunk == 1 else 4
+        ms = timed(lambda i: lof.gemm(A, B, spec), it)
+        key = f"gemm4096_{inp}_{name}_c{chunk}"
+        out[key + "_TFLOPs"] = round(flop / ms / 1e9, 3)
+        out[key + "_config"] = lof.gemm_config_name(cfg)
+        got = lof.gemm(A, B, spec).flatten()[sample_t].cpu().numpy()
+        t0 = time.perf_counter()
+        ref = oracle.gemm(Ah, Bh, oacc(chunk), samples=samples)
+        cpu_s = time.perf_counter() - t0
+        same = bool(np.array_equal(got.view(np.uint32), np.asarray(ref, np.float32).ravel().view(np.uint32)))
+        out[key + "_parity_4096_samples"] = same
+        out[key + "_cpu_oracle_GMACs"] = round(samples.size * N / cpu_s / 1e9, 4)
+        log(f"{key}: {ms:.2f} ms, {out[key + '_TFLOPs']} TFLOP/s ({out[key + '_config']}), "
+            f"sampled parity {same}, oracle {out[key + '_cpu_oracle_GMACs']} G emulated MAC/s")
+    del A, B, Ah, Bh
     out["gemm_cpu_oracle_threads"] = oracle.max_threads()
     out["gemm_roofline_peak_TFLOPs"] = VALU_FP32_PEAK_TFLOPS
 
@@ -418,12 +432,14 @@ def pack_codec(lof, torch, np, dev, out):
 
 def linear_steps(lof, torch, np, dev, out):
     """LoF_Linear forward + backward: X 8192x4096, W 4096x4096, MX-e4m3 operands, e5m2 SR-A
-    gradients, accumulators 5e10m chunk 32 and 10-bit mantissa chunk 1."""
+    gradients, accumulators 5e10m chunk 32, 10-bit mantissa chunk 1 and the layer's default
+    23-bit mantissa chunk 1."""
     e4m3 = lof.make_format(8, 3, 7)
     fp16 = lof.make_format(16, 10, 15)
     M, K, Nout = 8192, 4096, 4096
     for label, kw in (("5e10m_c32", dict(accum_format=fp16, accum_chunk=32)),
-                      ("mant10_c1", dict(accum_mant_bits=10, accum_chunk=1))):
+                      ("mant10_c1", dict(accum_mant_bits=10, accum_chunk=1)),
+                      ("mant23_c1", dict(accum_mant_bits=23, accum_chunk=1))):
         lin = lof.LoF_Linear(K, Nout, act_params=e4m3, weight_params=e4m3, bias=False,
                              scaling="mx", mx_block_size=32,
                              grad_spec=lof.LinearGradSpec(lof.make_format(8, 2, 15), 3, 8,
@@ -454,13 +470,16 @@ def linear_steps(lof, torch, np, dev, out):
     torch.cuda.empty_cache()
 
 
-def grouped_vs_separate(lof, torch, np, dev, out):
-    """64 uneven rounding jobs (own format and mode each) + 12 emulated GEMMs of MLP shapes,
-    one persistent launch vs per-job launches, both captured in HIP graphs.  GEMM operands
-    are pre-rounded to e4m3, accumulator 5e10m with chunk 32."""
+def c5_jobs(lof, torch, np, dev):
+    """The batched workload: 64 rounding jobs of log-uniform size 2^12 .. 2^24 elements, each
+    with its own format (exponent 2-8, mantissa 1-10 bits) and mode (RNE, round-to-odd,
+    round-to-nearest-odd, toward zero, SR-A with explicit random words), and 12 emulated GEMMs
+    with dimensions drawn from {1024, 2048, 4096, 8192}, operands N(0, 1/64) pre-rounded to
+    e4m3, accumulator 5e10m at chunk 32.  Seeded; returns (jobs, total GEMM flop)."""
     fp16 = lof.make_format(16, 10, 15)
     e4m3 = lof.make_format(8, 3, 7)
     rng = np.random.default_rng(5)
+    gen = torch.Generator(device=dev).manual_seed(55)
     jobs = []
     modes = [0, 12, 4, 1, 3]
     for i in range(64):
@@ -468,18 +487,25 @@ def grouped_vs_separate(lof, torch, np, dev, out):
         e, m = int(rng.integers(2, 9)), int(rng.integers(1, 11))
         f = lof.make_format(1 + e + m, m, (1 << (e - 1)) - 1)
         mode = modes[i % 5]
-        xi = torch.randn(nn, device=dev)
-        bt = (torch.randint(-2**31, 2**31 - 1, (nn,), dtype=torch.int32, device=dev)
-              if mode == 3 else None)
+        xi = torch.randn(nn, device=dev, generator=gen)
+        bt = (torch.randint(-2**31, 2**31 - 1, (nn,), dtype=torch.int32, device=dev,
+                            generator=gen) if mode == 3 else None)
         jobs.append(lof.RoundJob(xi, f, mode, min(8, 23 - m) if mode == 3 else 0, rand_bits=bt))
     dims = [1024, 2048, 4096, 8192]
     gemm_flop = 0.0
     for j in range(12):
         Mg, Ng, Kg = (int(rng.choice(dims)) for _ in range(3))
-        A = lof.virtual_round(torch.randn(Mg, Kg, device=dev) / 64, e4m3)
-        B = lof.virtual_round(torch.randn(Kg, Ng, device=dev) / 64, e4m3)
+        A = lof.virtual_round(torch.randn(Mg, Kg, device=dev, generator=gen) / 64, e4m3)
+        B = lof.virtual_round(torch.randn(Kg, Ng, device=dev, generator=gen) / 64, e4m3)
         jobs.append(lof.GemmJob(A, B, lof.AccumSpec(fmt=fp16, chunk=32)))
         gemm_flop += 2.0 * Mg * Ng * Kg
+    return jobs, gemm_flop
+
+
+def grouped_vs_separate(lof, torch, np, dev, out):
+    """The batched workload (c5_jobs): one persistent launch vs per-job launches, both
+    captured in HIP graphs."""
+    jobs, gemm_flop = c5_jobs(lof, torch, np, dev)
     cfg, tune_ms = lof.grouped.autotune(jobs, iters=1)
     plan = lof.GroupedPlan(jobs)
     gg = plan.capture_graph(separate=False)

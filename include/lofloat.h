@@ -181,8 +181,14 @@ typedef struct lof_gemm_desc {
  *   0x800   turns the MFMA configs' exact-sum leaves off;
  *   0x1000  turns their RTO leaves off;
  *   0x2000  turns their binary16-held chunk-1 leaves off;
- *   0x4000  turns off only the packed binary16 form of those leaves.
- * Chunk sums run on MFMA only where every partial sum of the chunk is exact in FP32, and the
+ *   0x4000  turns off only the packed binary16 form of those leaves;
+ *   0x8000  turns off the directed-rounding chunk-1 leaves (round-to-odd from the FMA
+ *           rounded down and up, for mantissa-only and binary16-shaped accumulators);
+ *   0x10000 turns off the bf16 MFMA chunk sums (exact chunks of 8 to 32 then sum on the
+ *           VALU chain; chunks of 2 and 4 always use the f32 MFMA).
+ * Chunk sums run on MFMA only where every partial sum of the chunk is exact in FP32 (chunks of
+ * 8 to 32 on the bf16 MFMAs, where the slice's values are all bfloat16 values; chunks of 2
+ * and 4 on the f32 MFMA), and the
  * short leaves only where they provably equal the reference leaf, both checked per tile and
  * k-slice at run time, so all configs give bit-identical results.  Replaces lo_float::LoF_gemm
  * (Lof_kernel.cu:96-154) and lo_float::Gemm::operator() (cutlass_gemms.cuh:1789-1822;
@@ -197,8 +203,10 @@ LOF_API int32_t lof_gemm_config_valid(int32_t config, int32_t chunk);
 /* Writes a short "BMxBN/TMxTN" style name; returns LOF_ERR_INVALID_ARGUMENT if out of range. */
 LOF_API lof_status lof_gemm_config_name(int32_t config, char* buf, int32_t buflen);
 
-/* Autotuner: times every tile config on this problem (desc->D is overwritten), records
- * the winner for problems of the same shape class and returns it. */
+/* Autotuner: times every tile config on this problem (desc->D is overwritten; C and beta are
+ * not used, and a C that overlaps D is rejected with LOF_ERR_INVALID_ARGUMENT), records the
+ * winner for problems of the same shape class and accumulator (kind, mantissa bits, format
+ * bias, chunk) and returns it.  The operands' values are not part of the key. */
 LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
                                      lof_stream_t stream, int32_t* best_config, float* best_ms);
 
@@ -212,9 +220,10 @@ LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
  *     unfold(x), the order of LoF_Conv2d's im2col + lof_gemm path (layers.py:687-737).
  *   LOF_CONV_NHWC: x N x H x W x C, w K x R x S x C (KRSC), y and c N x P x Q x K; the
  *     reduction index is (r, s, c) with c fastest.
- * Each output is the ascending-k leaf chain of lof_gemm_f32 over its reduction.  Only the
- * wave-tile configs run convolutions (chunk in {1, 2, 4, 8, 16, 32}); x must have fewer
- * than 2^31 elements.
+ * Each output is the ascending-k leaf chain of lof_gemm_f32 over its reduction.  Convolutions
+ * run on two wave tiles, configs 7 (64 x 64) and 6 (128 x 64) (others return
+ * LOF_ERR_UNSUPPORTED), with chunk in {1, 2, 4, 8, 16, 32}; x must have fewer than 2^30
+ * elements.
  * ------------------------------------------------------------------------------- */
 enum { LOF_CONV_NCHW = 0, LOF_CONV_NHWC = 1 };
 
@@ -232,8 +241,8 @@ typedef struct lof_conv2d_desc {
     float beta;
 } lof_conv2d_desc;
 
-/* y = alpha * conv(x, w) [+ beta * c].  config: -1 for the default wave tile, else a
- * wave-tile config index (lof_gemm_config_valid).  Replaces lo_float::LoF_conv2d
+/* y = alpha * conv(x, w) [+ beta * c].  config: -1 for the default wave tile, else 6 or 7,
+ * with lof_gemm_f32's testing bits.  Replaces lo_float::LoF_conv2d
  * (Lof_kernel.cu:156-228; NCHW/OIHW, alpha = 1 / (weight_scale * input_scale), beta = 0)
  * and lo_float::Conv2d::operator() (cutlass_conv.cuh:118-181; NHWC/KRSC).  The reference's
  * CUTLASS mainloop fixes its own k order; the order here is the one stated above. */

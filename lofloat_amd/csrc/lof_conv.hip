@@ -131,9 +131,22 @@ LOF_API lof_status lof_conv2d_f32(const lof_conv2d_desc* desc, int32_t config,
                          "%s: config %d: convolutions run on configs %d (mfma11) and %d (mfma21)",
                          what, cfg, kCfgMfma11, kCfgMfma21);
     }
-    // testing bits as in lof_gemm_f32: 0x100 general leaf path, 0x200 every chunk on VALU
-    if (config >= 0 && (config & 0x100)) set_fast_path(&g, false, 0);
-    if (config >= 0 && (config & 0x200)) g.mfma_mode = 1;
+    // testing bits as in lof_gemm_f32: 0x100 general leaf path, 0x200 every chunk on VALU,
+    // 0x800 / 0x1000 / 0x2000 / 0x4000 / 0x8000 the exact-sum / RTO / binary16-held /
+    // packed binary16 / directed-rounding leaves off, 0x10000 the bf16 MFMA chunk sums off
+    if (config >= 0) {
+        if (config & 0x800) g.sum_off |= 1;
+        if (config & 0x1000) g.sum_off |= 2;
+        if (config & 0x2000) g.sum_off |= 4;
+        if (config & 0x4000) g.sum_off |= 8;
+        if (config & 0x8000) g.sum_off |= 16;
+        if (config & 0x10000) g.mfma_bf16 = 0;
+        if (config & 0x100) {
+            set_fast_path(&g, false, 0);
+            g.dr_kind = 0;
+        }
+        if (config & 0x200) g.mfma_mode = 1;
+    }
     if (c.N == 0 || c.K == 0) return LOF_OK;
 
     g.tb = 0;

@@ -84,8 +84,9 @@ GemmKernel pick_kernel(const GemmConfig& c, const GemmArgs& g) {
     return c.mfma[l2][g.subn ? 1 : 0];
 }
 
-// autotuned choices, keyed by the problem's shape class
-using TuneKey = std::tuple<int, int, int, int, int, int, int>;
+// autotuned choices, keyed by the problem's shape class and accumulator (kind, mantissa
+// bits, format bias, chunk); not by the operands' values, which the tuner never inspects
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int>;
 std::mutex g_tune_mu;
 std::map<TuneKey, int> g_tuned;
 
@@ -96,8 +97,10 @@ int log2_bucket(int v) {
 }
 
 TuneKey tune_key(const lof_gemm_desc& d) {
+    const bool fmt = d.accum.kind == LOF_ACCUM_FORMAT;
     return TuneKey{log2_bucket(d.M), log2_bucket(d.N), log2_bucket(d.K),
-                   d.trans_a * 2 + d.trans_b, d.accum.kind, d.accum.chunk > 1 ? 1 : 0,
+                   d.trans_a * 2 + d.trans_b, d.accum.kind,
+                   fmt ? d.accum.fmt.mantissa_bits : d.accum.mant_bits, fmt ? d.accum.fmt.bias : 0,
                    d.accum.chunk};
 }
 
@@ -250,6 +253,14 @@ lof_status prepare_gemm(const lof_gemm_desc* d, GemmArgs* g) {
     g->sum_rto = (ac.kind == LOF_ACCUM_MANTISSA || g->sum_f16) ? 1 : 0;
     // a 10-bit mantissa-only accumulator rounds like binary16 on values scaled into its range
     g->sum_h10 = (ac.kind == LOF_ACCUM_MANTISSA && m == 10) ? 1 : 0;
+    // directed-rounding chunk-1 leaves: round-to-nearest-even into a mantissa-only accumulator
+    // of any width (a bitwise round of the RTO24 pattern) or a binary16-shaped format
+    g->dr_kind = ac.spec.mode != RNE ? 0 : ac.kind == LOF_ACCUM_MANTISSA ? 1 : g->sum_f16 ? 2 : 0;
+    g->dr_rf = 23 - m;
+    g->dr_half = g->dr_rf > 0 ? (1u << (g->dr_rf - 1)) - 1u : 0u;
+    g->dr_lsb = g->dr_rf > 0 ? 1u : 0u;
+    g->dr_mask = ~((1u << g->dr_rf) - 1u);
+    g->mfma_bf16 = 1;
     return LOF_OK;
 }
 
@@ -296,12 +307,18 @@ LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_s
     // an MFMA config on the VALU chain; 0x400 runs every chunk on MFMA without the exactness
     // test (results then differ where a chunk is inexact); 0x800 turns the MFMA tile's
     // exact-sum leaves off, 0x1000 its RTO leaves, 0x2000 its binary16-held chunk-1 leaves,
-    // 0x4000 only their packed binary16 form
+    // 0x4000 only their packed binary16 form, 0x8000 the directed-rounding chunk-1 leaves,
+    // 0x10000 the bf16 MFMA chunk sums (exact chunks of 8 to 32 then use the VALU chain)
     if (config >= 0 && (config & 0x800)) g.sum_off |= 1;
     if (config >= 0 && (config & 0x1000)) g.sum_off |= 2;
     if (config >= 0 && (config & 0x2000)) g.sum_off |= 4;
     if (config >= 0 && (config & 0x4000)) g.sum_off |= 8;
-    if (config >= 0 && (config & 0x100)) set_fast_path(&g, false, 0);
+    if (config >= 0 && (config & 0x8000)) g.sum_off |= 16;
+    if (config >= 0 && (config & 0x10000)) g.mfma_bf16 = 0;
+    if (config >= 0 && (config & 0x100)) {
+        set_fast_path(&g, false, 0);
+        g.dr_kind = 0;
+    }
     if (config >= 0 && (config & 0x200)) g.mfma_mode = 1;
     else if (config >= 0 && (config & 0x400)) g.mfma_mode = 2;
     return launch_gemm(g, resolve_gemm_config(*desc, cfg_bits), (hipStream_t)stream);
@@ -326,6 +343,19 @@ LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
     GemmArgs g;
     lof_status s = prepare_gemm(desc, &g);
     if (s != LOF_OK) return s;
+    if (desc->beta != 0.0f && desc->C && desc->M > 0 && desc->N > 0) {
+        // the timing launches write D; a source C inside D's rows would be lost
+        const float* c0 = desc->C;
+        const float* c1 = desc->C + (int64_t)(desc->M - 1) * desc->ldc + desc->N;
+        const float* d0 = desc->D;
+        const float* d1 = desc->D + (int64_t)(desc->M - 1) * desc->ldd + desc->N;
+        if (c0 < d1 && d0 < c1)
+            return set_error(LOF_ERR_INVALID_ARGUMENT,
+                             "lof_gemm_autotune: C overlaps D, which the timing runs overwrite; "
+                             "tune with a separate D");
+    }
+    g.beta = 0.0f;   // time the plain epilogue: C is never read
+    g.C = nullptr;
     if (iters < 1) iters = 1;
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t e0, e1;

@@ -112,10 +112,19 @@ struct GemmArgs {
     int32_t sum_rto;     // RTO leaves round this accumulator exactly (mantissa-only, binary16)
     int32_t sum_h10;     // mantissa-only with m = 10: binary16's significand, any exponent
     int32_t sum_off;     // testing: bit 0 exact-sum leaves off, bit 1 RTO leaves off,
-                         // bit 2 binary16-held chunk-1 leaves off, bit 3 their packed form
+                         // bit 2 binary16-held chunk-1 leaves off, bit 3 their packed form,
+                         // bit 4 the directed-rounding chunk-1 leaves
+    // directed-rounding chunk-1 leaves (any product, exact or not): 0 off, 1 mantissa-only
+    // accumulator (round-to-nearest-even at bit dr_rf of the pattern), 2 binary16-shaped
+    int32_t dr_kind;
+    int32_t dr_rf;       // 23 - mantissa bits
+    uint32_t dr_half;    // (1 << (dr_rf - 1)) - 1, 0 when dr_rf == 0
+    uint32_t dr_lsb;     // 1, 0 when dr_rf == 0 (rounding to 23 bits is the identity)
+    uint32_t dr_mask;    // ~((1 << dr_rf) - 1)
     int32_t vec;         // A, B 16-byte aligned with lda, ldb multiples of 4 (float4 loads)
     int32_t mfma_mode;   // MFMA tile: 0 exactness test, 1 every chunk on the VALU chain,
                          // 2 every chunk on MFMA (testing only: skips the test)
+    int32_t mfma_bf16;   // exact chunk sums of bfloat16-valued slices on the bf16 MFMAs
     ConvGeom cv;         // wave tiles only: B gathered from an activation (cv.conv != 0)
 };
 

@@ -40,6 +40,23 @@ namespace lofamd {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Two floats that are bfloat16 values (their low 16 bits are zero) as one packed pair: the
+// high halves of lo and hi.
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+    return __builtin_amdgcn_perm(__float_as_uint(hi), __float_as_uint(lo), 0x07060302u);
+}
+
+// p[0], p[stride], ..., p[7 * stride] (LDS floats holding bfloat16 values) as an MFMA operand
+__device__ __forceinline__ bf16x8 lds_bf16x8(const float* p, int stride) {
+    const u32x4 v = {bf16_pair(p[0], p[stride]), bf16_pair(p[2 * stride], p[3 * stride]),
+                     bf16_pair(p[4 * stride], p[5 * stride]), bf16_pair(p[6 * stride], p[7 * stride])};
+    return __builtin_bit_cast(bf16x8, v);
+}
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
@@ -143,6 +160,92 @@ __device__ __forceinline__ float leaf_rto(float p, float na, const GemmArgs& g) 
     return leaf_exact<F16>(__uint_as_float(e == 0.0f ? tb : odd), g);
 }
 
+// RD / RU: a[o] * b + c[o] rounded down and up, for 16 outputs.  The wave's FP32 rounding
+// mode (MODE.FP_ROUND[1:0]: 2 toward -inf, 1 toward +inf, 0 nearest even) is switched only
+// inside this one asm block, which ends in round-to-nearest-even again, so no FP operation
+// the compiler schedules ever runs under a directed mode.  The FMA is correctly rounded in
+// every mode, FP32 subnormals included (denormals are not flushed), so rd / ru are the two
+// directed roundings of the exact a * b + c.
+__device__ __forceinline__ void fma_down_up16(const float (&a)[16], float b, const float (&c)[16],
+                                              float (&rd)[16], float (&ru)[16]) {
+    asm volatile(
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 2\n\t"
+        "s_nop 1\n\t"
+        "v_fma_f32 %0, %32, %48, %49\n\t"
+        "v_fma_f32 %1, %33, %48, %50\n\t"
+        "v_fma_f32 %2, %34, %48, %51\n\t"
+        "v_fma_f32 %3, %35, %48, %52\n\t"
+        "v_fma_f32 %4, %36, %48, %53\n\t"
+        "v_fma_f32 %5, %37, %48, %54\n\t"
+        "v_fma_f32 %6, %38, %48, %55\n\t"
+        "v_fma_f32 %7, %39, %48, %56\n\t"
+        "v_fma_f32 %8, %40, %48, %57\n\t"
+        "v_fma_f32 %9, %41, %48, %58\n\t"
+        "v_fma_f32 %10, %42, %48, %59\n\t"
+        "v_fma_f32 %11, %43, %48, %60\n\t"
+        "v_fma_f32 %12, %44, %48, %61\n\t"
+        "v_fma_f32 %13, %45, %48, %62\n\t"
+        "v_fma_f32 %14, %46, %48, %63\n\t"
+        "v_fma_f32 %15, %47, %48, %64\n\t"
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 1\n\t"
+        "s_nop 1\n\t"
+        "v_fma_f32 %16, %32, %48, %49\n\t"
+        "v_fma_f32 %17, %33, %48, %50\n\t"
+        "v_fma_f32 %18, %34, %48, %51\n\t"
+        "v_fma_f32 %19, %35, %48, %52\n\t"
+        "v_fma_f32 %20, %36, %48, %53\n\t"
+        "v_fma_f32 %21, %37, %48, %54\n\t"
+        "v_fma_f32 %22, %38, %48, %55\n\t"
+        "v_fma_f32 %23, %39, %48, %56\n\t"
+        "v_fma_f32 %24, %40, %48, %57\n\t"
+        "v_fma_f32 %25, %41, %48, %58\n\t"
+        "v_fma_f32 %26, %42, %48, %59\n\t"
+        "v_fma_f32 %27, %43, %48, %60\n\t"
+        "v_fma_f32 %28, %44, %48, %61\n\t"
+        "v_fma_f32 %29, %45, %48, %62\n\t"
+        "v_fma_f32 %30, %46, %48, %63\n\t"
+        "v_fma_f32 %31, %47, %48, %64\n\t"
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 0\n\t"
+        "s_nop 1"
+        : "=&v"(rd[0]), "=&v"(rd[1]), "=&v"(rd[2]), "=&v"(rd[3]), "=&v"(rd[4]), "=&v"(rd[5]),
+          "=&v"(rd[6]), "=&v"(rd[7]), "=&v"(rd[8]), "=&v"(rd[9]), "=&v"(rd[10]), "=&v"(rd[11]),
+          "=&v"(rd[12]), "=&v"(rd[13]), "=&v"(rd[14]), "=&v"(rd[15]),
+          "=&v"(ru[0]), "=&v"(ru[1]), "=&v"(ru[2]), "=&v"(ru[3]), "=&v"(ru[4]), "=&v"(ru[5]),
+          "=&v"(ru[6]), "=&v"(ru[7]), "=&v"(ru[8]), "=&v"(ru[9]), "=&v"(ru[10]), "=&v"(ru[11]),
+          "=&v"(ru[12]), "=&v"(ru[13]), "=&v"(ru[14]), "=&v"(ru[15])
+        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]),
+          "v"(a[7]), "v"(a[8]), "v"(a[9]), "v"(a[10]), "v"(a[11]), "v"(a[12]), "v"(a[13]),
+          "v"(a[14]), "v"(a[15]), "v"(b),
+          "v"(c[0]), "v"(c[1]), "v"(c[2]), "v"(c[3]), "v"(c[4]), "v"(c[5]), "v"(c[6]),
+          "v"(c[7]), "v"(c[8]), "v"(c[9]), "v"(c[10]), "v"(c[11]), "v"(c[12]), "v"(c[13]),
+          "v"(c[14]), "v"(c[15]));
+}
+
+// Directed-rounding leaves of 16 outputs: acc[o] <- round(RTO24(a[o] * b + acc[o])) for any
+// operands, the products exact or not.  This is round_to_odd_fma's own construction
+// (cutlass_gemms.cuh:74-80) from the directed roundings rd / ru of x = a * b + acc: the
+// truncation rz = (rd >= 0 ? rd : ru) with its last bit set when rd != ru.  So an exact x is
+// rd, an exact zero with roundTowardNegative's sign ((+0) + (+0) = +0, any other exact zero
+// -0), and an overflowing x gives FLT_MAX | 1 with its sign.
+// round() is round-to-nearest-even at bit dr_rf of the pattern (a mantissa-only accumulator:
+// bitwise virtual_round, lo_float.h:2716-2756) or, F16, the binary16 conversion (a
+// binary16-shaped format, for sums the caller has bounded below 2^15).
+template <bool F16>
+__device__ __forceinline__ void leaves_dr16(const float (&a)[16], float b, float (&acc)[16],
+                                            const GemmArgs& g) {
+    float rd[16], ru[16];
+    fma_down_up16(a, b, acc, rd, ru);
+#pragma unroll
+    for (int o = 0; o < 16; ++o) {
+        const uint32_t d = __float_as_uint(rd[o]), u = __float_as_uint(ru[o]);
+        const uint32_t t = (rd[o] >= 0.0f ? d : u) | (rd[o] != ru[o] ? 1u : 0u);
+        if constexpr (F16)
+            acc[o] = (float)(_Float16)__uint_as_float(t);
+        else
+            acc[o] = __uint_as_float((t + g.dr_half + ((t >> g.dr_rf) & g.dr_lsb)) & g.dr_mask);
+    }
+}
+
 // Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
 // results are committed where valid, outputs needing the general path are recorded in a bit
 // mask, and one wave-uniform branch runs the general path for them (acc still holds their
@@ -234,6 +337,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // per-value lowest-bit statistics for the short chunks, where the exact-sum leaves
     // carry most of the work (measured: they cost more than they win from chunk 8 up)
     constexpr bool kKey = CHUNK <= 4;
+    // exact chunk sums on the bf16 MFMAs (chunks 8-32) or the f32 MFMA (chunks 2, 4)
+    constexpr bool kBf16 = CHUNK >= 8;
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -461,6 +566,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // value is a binary16 value; left at values no test passes when a slice is zero
         int bLsb = -1000, bTop = 1000, bW = 99;
         bool a16 = false;
+        // chunk > 1: every A and B value of the slice is a bfloat16 value (<= 8 significant
+        // bits; FP32-subnormal values excluded), so exact chunk sums can use the bf16 MFMAs
+        bool bf = false;
         {
             uint32_t amA = 0u, mnA = 0xFFFFFFFFu, orA = 0u, amB = 0u, mnB = 0xFFFFFFFFu, orB = 0u;
             uint32_t kyA = 0xFFFFFFFFu, kyB = 0xFFFFFFFFu;
@@ -503,6 +611,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     const int top = LOG2C + eA + eB + 2;
                     exact = normA && normB && L >= -126 && top <= L + 24 && top <= 127;
                     sTop = top;
+                    bf = g.mfma_bf16 && normA && normB &&
+                         __builtin_ctz((orA & 0x7FFFFFu) | 0x800000u) >= 16 &&
+                         __builtin_ctz((orB & 0x7FFFFFu) | 0x800000u) >= 16;
                 }
                 sL = L;
                 sfin = true;
@@ -551,10 +662,17 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // When every A value and every scaled B value is a binary16 value as well, the
         // binary16 FMA h <- binary16(-a (b 2^hs) + h) computes the same leaf, two outputs'
         // worth per packed instruction.
-        bool sum_ok = false, rto_ok = false, h_ok = false, pk_ok = false;
+        //
+        // Chunk 1, products exact or not: the directed-rounding leaves (leaves_dr16) round the
+        // reference's RTO24 sum exactly for a mantissa-only accumulator (any sum, any width)
+        // and for a binary16-shaped one while every sum stays below 2^15; a lane whose
+        // accumulators hold a NaN keeps the general leaf.
+        bool sum_ok = false, rto_ok = false, h_ok = false, pk_ok = false, dr_ok = false;
         int hs = 0;
-        if (sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
-            (g.sum_off & 3) != 3) {
+        const bool try_fast = sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
+                              (g.sum_off & 3) != 3;
+        const bool try_dr = CHUNK == 1 && sfin && g.dr_kind != 0 && !(g.sum_off & 16);
+        if (try_fast || try_dr) {
             uint32_t amax = 0u, mnk = 0xFFFFFFFFu;
 #pragma unroll
             for (int bi = 0; bi < WMB; ++bi)
@@ -577,10 +695,13 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             const float bnd = __builtin_fmaf(g.sum_g, __builtin_fmaf(g.sum_n, pp, pa), pp) *
                               1.0000152587890625f;   // 1 + 2^-16
             const int S = (int)(__float_as_uint(bnd) >> 23) - 126;
+            if (try_fast) {
             sum_ok = exact && sL >= g.sum_lreq && S <= min(sL + 24, g.sum_ehi) &&
                      sTop <= Lf + 23 && Lf >= -126 && !(g.sum_off & 1);
             rto_ok = !sum_ok && g.sum_rto && S <= g.sum_ehi && !(g.sum_off & 2);
+            }
             if constexpr (CHUNK == 1) {
+              if (try_fast) {
                 if (sum_ok && !(g.sum_off & 4)) {
                     if (g.sum_f16) {
                         h_ok = true;
@@ -599,12 +720,15 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     pk_ok = true;
                     hs = 0;
                 }
+              }
+              dr_ok = try_dr && !sum_ok && !pk_ok && amax <= 0x7F800000u &&
+                      (g.dr_kind == 1 || S <= g.sum_ehi);
             }
         }
 
         const int kmax = min(BK, g.K - k0);
         if constexpr (CHUNK == 1) {
-          set_form(sum_ok || rto_ok);
+          set_form(sum_ok || pk_ok || (rto_ok && !dr_ok));
           // nacc <- -round(a * b + acc): exact sums (fma(-a, b, nacc) is -(a * b + acc)
           // exactly) or RTO leaves on the exact product a * b
           auto leaves_c1 = [&](auto f16, auto rto) {
@@ -740,6 +864,32 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
           } else if (sum_ok) {
             if (g.sum_f16) leaves_c1(std::true_type{}, std::false_type{});
             else leaves_c1(std::false_type{}, std::false_type{});
+          } else if (dr_ok) {   // acc <- round(RTO24(a * b + acc)), plain form
+            auto dr_leaves = [&](auto f16) {
+              constexpr bool F16 = decltype(f16)::value;
+              for (int kk = 0; kk < kmax; ++kk) {
+                float4 a4[WMB][4];
+                float b[WNB];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq)
+                        a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj) b[bj] = Bs[kk * LDB_S + bn0 + bj * 32 + l32];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi) {
+                    const float av[16] = {a4[bi][0].x, a4[bi][0].y, a4[bi][0].z, a4[bi][0].w,
+                                          a4[bi][1].x, a4[bi][1].y, a4[bi][1].z, a4[bi][1].w,
+                                          a4[bi][2].x, a4[bi][2].y, a4[bi][2].z, a4[bi][2].w,
+                                          a4[bi][3].x, a4[bi][3].y, a4[bi][3].z, a4[bi][3].w};
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj) leaves_dr16<F16>(av, b[bj], nacc[bi][bj], g);
+                }
+              }
+            };
+            if (g.dr_kind == 2) dr_leaves(std::true_type{});
+            else dr_leaves(std::false_type{});
           } else if (rto_ok) {
             if (g.sum_f16) leaves_c1(std::true_type{}, std::true_type{});
             else leaves_c1(std::false_type{}, std::true_type{});
@@ -797,32 +947,89 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         for (int q = 0; q < BK / CHUNK; ++q) {
             const int kb = q * CHUNK;
             if (kb >= kmax) break;
-            if (exact) {
-                float a[CHUNK / 2][WMB], b[CHUNK / 2][WNB];
-#pragma unroll
-                for (int kp = 0; kp < CHUNK / 2; ++kp) {
-                    const int kk = kb + 2 * kp + lhi;
-#pragma unroll
-                    for (int bi = 0; bi < WMB; ++bi) a[kp][bi] = As[kk * LDA_S + am0 + bi * 32 + l32];
-#pragma unroll
-                    for (int bj = 0; bj < WNB; ++bj) b[kp][bj] = Bs[kk * LDB_S + bn0 + bj * 32 + l32];
-                }
+            // Chunks of 8 and more sum on the bf16 MFMAs only (an exact slice whose values are
+            // not all bfloat16 values takes the chain); chunks of 2 and 4 on the f32 MFMA.  One
+            // kernel holding both MFMA kinds spills heavily at three waves per SIMD.
+            if (exact && (!kBf16 || bf || g.mfma_mode == 2)) {
 #pragma unroll
                 for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
                     for (int bj = 0; bj < WNB; ++bj)
 #pragma unroll
                         for (int r = 0; r < 16; ++r) part[bi][bj][r] = 0.0f;
-                // all of the chunk's LDS reads are issued before its MFMAs
-                __builtin_amdgcn_sched_barrier(0);
+                if constexpr (kBf16) {
+                    // Every A and B value of the slice is a bfloat16 value and every partial sum of
+                    // the chunk is exact in FP32, so the bf16 MFMAs' products and sums are exact
+                    // too: 16 k per v_mfma_f32_32x32x16_bf16 (lane l: row / column l % 32, k
+                    // 8 (l / 32) .. + 7), a chunk of 8 on one v_mfma_f32_32x32x8_bf16 (k
+                    // 4 (l / 32) .. + 3).
+                    if constexpr (CHUNK >= 16) {
 #pragma unroll
-                for (int kp = 0; kp < CHUNK / 2; ++kp)
+                        for (int kq = 0; kq < CHUNK / 16; ++kq) {
+                            __builtin_amdgcn_sched_barrier(0);   // one k group's operands live at a time
+                            const int kk = kb + 16 * kq + 8 * lhi;
+                            bf16x8 av[WMB], bv[WNB];
 #pragma unroll
-                    for (int bi = 0; bi < WMB; ++bi)
+                            for (int bi = 0; bi < WMB; ++bi)
+                                av[bi] = lds_bf16x8(&As[kk * LDA_S + am0 + bi * 32 + l32], LDA_S);
+#pragma unroll
+                            for (int bj = 0; bj < WNB; ++bj)
+                                bv[bj] = lds_bf16x8(&Bs[kk * LDB_S + bn0 + bj * 32 + l32], LDB_S);
+#pragma unroll
+                            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                                for (int bj = 0; bj < WNB; ++bj)
+                                    part[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                        av[bi], bv[bj], part[bi][bj], 0, 0, 0);
+                        }
+                    } else {
+                        static_assert(CHUNK == 8, "one 32x32x8 MFMA per chunk");
+                        float ea[4][WMB], eb[4][WNB];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int kk = kb + 4 * lhi + j;
+#pragma unroll
+                            for (int bi = 0; bi < WMB; ++bi) ea[j][bi] = As[kk * LDA_S + am0 + bi * 32 + l32];
+#pragma unroll
+                            for (int bj = 0; bj < WNB; ++bj) eb[j][bj] = Bs[kk * LDB_S + bn0 + bj * 32 + l32];
+                        }
+                        bf16x4 av[WMB], bv[WNB];
+#pragma unroll
+                        for (int bi = 0; bi < WMB; ++bi)
+                            av[bi] = __builtin_bit_cast(bf16x4, u32x2{bf16_pair(ea[0][bi], ea[1][bi]),
+                                                                      bf16_pair(ea[2][bi], ea[3][bi])});
 #pragma unroll
                         for (int bj = 0; bj < WNB; ++bj)
-                            part[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x2f32(
-                                a[kp][bi], b[kp][bj], part[bi][bj], 0, 0, 0);
+                            bv[bj] = __builtin_bit_cast(bf16x4, u32x2{bf16_pair(eb[0][bj], eb[1][bj]),
+                                                                      bf16_pair(eb[2][bj], eb[3][bj])});
+#pragma unroll
+                        for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < WNB; ++bj)
+                                part[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(
+                                    av[bi], bv[bj], part[bi][bj], 0, 0, 0);
+                    }
+                } else {
+                    float a[CHUNK / 2][WMB], b[CHUNK / 2][WNB];
+#pragma unroll
+                    for (int kp = 0; kp < CHUNK / 2; ++kp) {
+                        const int kk = kb + 2 * kp + lhi;
+#pragma unroll
+                        for (int bi = 0; bi < WMB; ++bi) a[kp][bi] = As[kk * LDA_S + am0 + bi * 32 + l32];
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj) b[kp][bj] = Bs[kk * LDB_S + bn0 + bj * 32 + l32];
+                    }
+                    // all of the chunk's LDS reads are issued before its MFMAs
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int kp = 0; kp < CHUNK / 2; ++kp)
+#pragma unroll
+                        for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                            for (int bj = 0; bj < WNB; ++bj)
+                                part[bi][bj] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                                    a[kp][bi], b[kp][bj], part[bi][bj], 0, 0, 0);
+                }
 #pragma unroll
                 for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll

@@ -55,9 +55,9 @@ __device__ __forceinline__ float round_one(float x, const RoundArgs& a, uint32_t
     return __uint_as_float(y);
 }
 
-template <int MODE, bool MANT, int RSRC, int VPT>
+// One tile = TPB lanes x VPT float4 slots, TPB * VPT * 4 consecutive floats.
+template <int MODE, bool MANT, int RSRC, int VPT, int TPB = 256>
 __device__ __forceinline__ void round_tile(const RoundArgs& a, int64_t tile, int tid) {
-    constexpr int TPB = 256;
     if (a.vec_ok) {
         const int64_t nvec = a.n >> 2;
         const int64_t v0 = tile * (TPB * VPT) + tid;
@@ -160,18 +160,20 @@ __device__ __forceinline__ void mx_round_lane(float (&x)[EPL], const uint32_t (&
     }
 }
 
-// One MX tile = 2048 consecutive floats.  EPL = floats per lane: 8 (two 16-byte accesses,
-// block_size >= 8) or 4 (block_size 4, two passes).  A block is block_size / EPL
-// consecutive lanes.
+// One MX tile = TPB * 8 consecutive floats (2048 for 256 lanes).  EPL = floats per lane: 8
+// (two 16-byte accesses, block_size >= 8) or 4 (block_size 4, two passes).  A block is
+// block_size / EPL consecutive lanes of one wave (TPB >= 64, block_size <= 256).
 constexpr int64_t kMxTileElems = 2048;
 
-template <int MODE, int RSRC, int EPL, bool SRNE>
+template <int MODE, int RSRC, int EPL, bool SRNE, int TPB = 256>
 __device__ __forceinline__ void mx_tile(const MxArgs& a, int64_t tile, int tid) {
-    constexpr int PASSES = (int)(kMxTileElems / (256 * EPL));
+    static_assert(TPB >= 64 && TPB % 64 == 0, "whole waves");
+    constexpr int64_t TILE = (int64_t)TPB * 8;
+    constexpr int PASSES = 8 / EPL;
     const int lanes = (1 << a.lanes_log2) * 4 / EPL;
 #pragma unroll
     for (int pass = 0; pass < PASSES; ++pass) {
-        const int64_t e0 = tile * kMxTileElems + (int64_t)pass * (256 * EPL) + (int64_t)tid * EPL;
+        const int64_t e0 = tile * TILE + (int64_t)pass * (TPB * EPL) + (int64_t)tid * EPL;
         float x[EPL];
         uint32_t u[EPL];
         if (a.vec_ok) {  // 16-byte aligned, n % 4 == 0

@@ -131,6 +131,31 @@ def test_alpha_beta_and_forced_paths(dev):
         assert_bits(y2, ref, f"forced path {bits:#x}")
 
 
+@pytest.mark.parametrize("kind", ["bf16", "gauss"])
+def test_chunk1_binary16_accumulators(dev, kind):
+    """Chunk 1 with binary16-shaped accumulators (saturating and extended 5e10m): bf16-like
+    8e7m inputs that binary16 holds (the packed binary16 leaves), one tiny input value whose
+    slices take the round-to-odd leaves, and N(0, 1) inputs whose products FP32 cannot hold
+    (the directed-rounding leaves); bit for bit against the oracle with every leaf kind on,
+    and with the exact-sum / RTO / binary16-held / packed / directed-rounding leaves switched
+    off (0x800 .. 0x8000)."""
+    import lofloat_amd as lof
+    x, w = _inputs("gauss", (2, 8, 11, 11), (24, 8, 3, 3), seed=77)
+    if kind == "bf16":
+        x = oracle.virtual_round(x / 8, (16, 7, 127, 1, 0, 0)).reshape(x.shape)
+        w = oracle.virtual_round(w / 8, (16, 7, 127, 1, 0, 0)).reshape(w.shape)
+        x[1, 3, 5, 5] = np.float32(2.0 ** -30)
+    for fmt in ((16, 10, 15, 1, 0, 0), (16, 10, 15, 0, 0, 0)):
+        spec = lof.AccumSpec(fmt=lof.make_format(16, 10, 15, saturating=bool(fmt[3])), chunk=1)
+        acc = oracle.accum_fmt(fmt, 1)
+        y, ref = _run(dev, x, w, spec, acc, (1, 1), (1, 1), (1, 1), config=CFG_64)
+        assert_bits(y, ref, f"{kind} {fmt} chunk 1")
+        for bits in (0x800, 0x1000, 0x2000, 0x4000, 0x8000, 0xF800):
+            for cfg in (CFG_64, CFG_128):
+                y2, _ = _run(dev, x, w, spec, acc, (1, 1), (1, 1), (1, 1), config=cfg | bits)
+                assert_bits(y2, ref, f"{kind} {fmt} cfg {cfg} bits {bits:#x}")
+
+
 def test_empty_and_errors(dev):
     import lofloat_amd as lof
     x = torch.randn(0, 3, 8, 8, device=dev)

@@ -119,6 +119,8 @@ def test_mfma_chunk_paths(dev, kind, chunk):
             assert_bits(_run(lof, A, B, acc, dev, config=cfg), ref, f"{kind} c{chunk} cfg {cfg}")
             assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x200), ref,
                         f"{kind} c{chunk} cfg {cfg} chain")
+            assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x10000), ref,
+                        f"{kind} c{chunk} cfg {cfg} bf16 MFMAs off")
 
 
 @pytest.mark.parametrize("cfg", MFMA_CONFIGS)
@@ -128,10 +130,18 @@ def test_mfma_forced(dev, cfg):
     MFMA operand and result layouts on every tile position).  The automatic choice must
     match the oracle on general FP32 inputs as well."""
     import lofloat_amd as lof
-    for acc, oacc in (_accums(lof, 10, 32), _accums(lof, fmt=FP16_SAT, chunk=4)):
+    for acc, oacc in (_accums(lof, 10, 32), _accums(lof, fmt=FP16_SAT, chunk=4),
+                      _accums(lof, 7, 16), _accums(lof, fmt=FP16_EXT, chunk=8),
+                      _accums(lof, 10, 2)):
         A, B = _mats(256, 192, 320, seed=cfg, kind="e4m3_small")
-        assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x400), _ref(A, B, oacc),
-                    "forced exact")
+        ref = _ref(A, B, oacc)
+        # e4m3 values are bfloat16 values: forced chunk sums on the bf16 MFMAs (chunks 8-32)
+        # or the f32 MFMA (chunks 2, 4), which pins their operand and result layouts; 0x10000
+        # (bf16 MFMAs off) must not change a forced run
+        assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x400), ref,
+                    f"forced exact chunk {acc.chunk}")
+        assert_bits(_run(lof, A, B, acc, dev, config=cfg | 0x10400), ref,
+                    f"forced exact, bf16 flag off, chunk {acc.chunk}")
         A, B = _mats(256, 192, 320, seed=cfg, kind="normal")
         assert_bits(_run(lof, A, B, acc, dev, config=cfg), _ref(A, B, oacc),
                     "auto on inexact inputs")
@@ -382,6 +392,47 @@ def test_rto_leaves(dev, chunk):
             for bits in (0, 0x1000, 0x1800):
                 assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
                             f"rto chunk {chunk} config {cfg} bits {bits:#x}")
+
+
+@pytest.mark.parametrize("kind", ["uniform", "mx", "wide", "overflow", "tiny"])
+def test_directed_rounding_leaves(dev, kind):
+    """Chunk 1 with products that FP32 cannot hold (24-bit significands): the directed-rounding
+    leaves (round-to-odd from the FMA rounded down and up) for mantissa-only accumulators of
+    every width and for binary16-shaped formats, against the oracle on every wave tile, and
+    the same tiles with those leaves off (0x8000).  Kinds: FP32 uniform(-1, 1) operands (the
+    reference harness's lof_gemm inputs), MX-rounded e4m3 operands whose overflowing
+    elements pass through unrounded, magnitudes across the FP32 range, sums that overflow to
+    FLT_MAX | 1 and inf, and products in the FP32 subnormal range."""
+    import lofloat_amd as lof
+    g = np.random.default_rng({"uniform": 1, "mx": 2, "wide": 3, "overflow": 4, "tiny": 5}[kind])
+    M, N, K = 80, 96, 192
+    if kind == "uniform":
+        A = g.uniform(-1, 1, (M, K)).astype(np.float32)
+        B = g.uniform(-1, 1, (K, N)).astype(np.float32)
+    elif kind == "mx":
+        A = oracle.virtual_mx_round((g.standard_normal(M * K) * 0.02).astype(np.float32), 32,
+                                    E4M3, (8, 0, 127, 0, 0, 1)).reshape(M, K)
+        Bt = oracle.virtual_mx_round((g.standard_normal(N * K) * 0.02).astype(np.float32), 32,
+                                     E4M3, (8, 0, 127, 0, 0, 1)).reshape(N, K)
+        B = np.ascontiguousarray(Bt.T)
+    elif kind == "wide":
+        A, B = _mats(M, N, K, seed=31, kind="wide")
+    elif kind == "overflow":
+        A = (g.standard_normal((M, K)) * 2.0 ** 62).astype(np.float32)
+        B = (g.standard_normal((K, N)) * 2.0 ** 64).astype(np.float32)
+    else:
+        A = (g.standard_normal((M, K)) * 2.0 ** -70).astype(np.float32)
+        B = (g.standard_normal((K, N)) * 2.0 ** -62).astype(np.float32)
+    accs = [_accums(lof, m) for m in (0, 5, 10, 19, 20, 21, 22, 23)]
+    accs += [_accums(lof, fmt=FP16_SAT), _accums(lof, fmt=FP16_EXT)]
+    for acc, oacc in accs:
+        ref = _ref(A, B, oacc)
+        what = f"{kind} m={acc.mant_bits if acc.fmt is None else 'fp16'}"
+        for cfg in MFMA_CONFIGS:
+            for bits in (0, 0x8000):
+                assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
+                            f"{what} config {cfg} bits {bits:#x}")
+        assert_bits(_run(lof, A, B, acc, dev), ref, f"{what} default config")
 
 
 def test_alpha_and_reference_entry(dev):

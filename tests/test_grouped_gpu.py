@@ -137,3 +137,27 @@ def test_gemm_autotune(dev):
     assert 0 <= cfg < lof.gemm_num_configs() and ms > 0
     ref = lof.gemm(A, B, acc, config=0)
     assert torch.equal(lof.gemm(A, B, acc), ref)  # every config gives the same bits
+
+
+def test_gemm_autotune_keeps_aliased_source(dev):
+    """lof_gemm_autotune refuses a source C that overlaps D (its timing runs overwrite D), so
+    the caller's C survives, and a later lof_gemm_f32 with C == D still reads it."""
+    import ctypes
+    import lofloat_amd as lof
+    from lofloat_amd import _lib
+    g = np.random.default_rng(8)
+    A = (g.standard_normal((64, 48)) / 8).astype(np.float32)
+    B = (g.standard_normal((48, 40)) / 8).astype(np.float32)
+    C = g.standard_normal((64, 40)).astype(np.float32)
+    Ad, Bd, Dd = (torch.from_numpy(v).to(dev) for v in (A, B, C))
+    acc = lof.AccumSpec(mant_bits=10, chunk=8)
+    desc = _lib.GemmDesc(64, 40, 48, Ad.data_ptr(), 48, 0, Bd.data_ptr(), 40, 0, Dd.data_ptr(),
+                         40, 1.0, acc.c_accum(), Dd.data_ptr(), 40, 0.5)
+    best, ms = ctypes.c_int32(-1), ctypes.c_float(0)
+    st = _lib.lib.lof_gemm_autotune(ctypes.byref(desc), 2, None, ctypes.byref(best),
+                                    ctypes.byref(ms))
+    torch.cuda.synchronize()
+    assert st == 1 and b"overlaps" in _lib.lib.lof_last_error()
+    assert np.array_equal(Dd.cpu().numpy(), C)
+    y = lof.gemm(Ad, Bd, acc, out=Dd, C=Dd, beta=0.5).cpu().numpy()
+    assert_bits(y, oracle.gemm(A, B, oracle.accum_mant(10, 8), C=C, beta=0.5), "C == D after autotune")
