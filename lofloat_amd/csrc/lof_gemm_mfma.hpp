@@ -222,11 +222,14 @@ __device__ __forceinline__ void fma_down_up16(const float (&a)[16], float b, con
 }
 
 // Directed-rounding leaves of 16 outputs: acc[o] <- round(RTO24(a[o] * b + acc[o])) for any
-// operands, the products exact or not.  This is round_to_odd_fma's own construction
+// operands, the products exact or not.  This is round_to_odd_fma's construction
 // (cutlass_gemms.cuh:74-80) from the directed roundings rd / ru of x = a * b + acc: the
 // truncation rz = (rd >= 0 ? rd : ru) with its last bit set when rd != ru.  So an exact x is
 // rd, an exact zero with roundTowardNegative's sign ((+0) + (+0) = +0, any other exact zero
-// -0), and an overflowing x gives FLT_MAX | 1 with its sign.
+// -0), and an overflowing x gives FLT_MAX | 1 with its sign.  When rd != ru, rd and ru have
+// the same sign bit (FP32 subnormals are kept, so only an exact zero straddles it) and the
+// truncation is the one of smaller magnitude, i.e. the smaller bit pattern: rz | 1 is
+// min(rd, ru) | 1 on the patterns (the larger pattern is +inf / -inf on overflow).
 // round() is round-to-nearest-even at bit dr_rf of the pattern (a mantissa-only accumulator:
 // bitwise virtual_round, lo_float.h:2716-2756) or, F16, the binary16 conversion (a
 // binary16-shaped format, for sums the caller has bounded below 2^15).
@@ -238,11 +241,12 @@ __device__ __forceinline__ void leaves_dr16(const float (&a)[16], float b, float
 #pragma unroll
     for (int o = 0; o < 16; ++o) {
         const uint32_t d = __float_as_uint(rd[o]), u = __float_as_uint(ru[o]);
-        const uint32_t t = (rd[o] >= 0.0f ? d : u) | (rd[o] != ru[o] ? 1u : 0u);
+        const uint32_t t = rd[o] == ru[o] ? d : (min(d, u) | 1u);
         if constexpr (F16)
             acc[o] = (float)(_Float16)__uint_as_float(t);
-        else
-            acc[o] = __uint_as_float((t + g.dr_half + ((t >> g.dr_rf) & g.dr_lsb)) & g.dr_mask);
+        else   // round to nearest even at bit dr_rf: the kept lsb (none when dr_rf is 0)
+            acc[o] = __uint_as_float((t + g.dr_half + __builtin_amdgcn_ubfe(t, g.dr_rf, g.dr_lsb)) &
+                                     g.dr_mask);
     }
 }
 
