@@ -182,8 +182,10 @@ typedef struct lof_gemm_desc {
  *   0x1000  turns their RTO leaves off;
  *   0x2000  turns their binary16-held chunk-1 leaves off;
  *   0x4000  turns off only the packed binary16 form of those leaves;
- *   0x8000  turns off the directed-rounding chunk-1 leaves (round-to-odd from the FMA
- *           rounded down and up, for mantissa-only and binary16-shaped accumulators);
+ *   0x8000  turns off the directed-rounding leaves (round-to-odd from the sum rounded down
+ *           and up, for mantissa-only and binary16-shaped accumulators at chunk 1, and on
+ *           the 64x64 and 64x128 wave tiles for the sums of chunks 8 to 32 into
+ *           mantissa-only accumulators of 20 or more bits);
  *   0x10000 turns off the bf16 MFMA chunk sums (exact chunks of 8 to 32 then sum on the
  *           VALU chain; chunks of 2 and 4 always use the f32 MFMA).
  * Chunk sums run on MFMA only where every partial sum of the chunk is exact in FP32 (chunks of
@@ -203,7 +205,8 @@ LOF_API int32_t lof_gemm_config_valid(int32_t config, int32_t chunk);
 /* Writes a short "BMxBN/TMxTN" style name; returns LOF_ERR_INVALID_ARGUMENT if out of range. */
 LOF_API lof_status lof_gemm_config_name(int32_t config, char* buf, int32_t buflen);
 
-/* Autotuner: times every tile config on this problem (desc->D is overwritten; C and beta are
+/* Autotuner: times the candidate tile configs on this problem -- the wave tiles for chunks
+ * they take, else the VALU tiles -- (desc->D is overwritten; C and beta are
  * not used, and a C that overlaps D is rejected with LOF_ERR_INVALID_ARGUMENT), records the
  * winner for problems of the same shape class and accumulator (kind, mantissa bits, format
  * bias, chunk) and returns it.  The operands' values are not part of the key. */

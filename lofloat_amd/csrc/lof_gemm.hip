@@ -26,12 +26,12 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
 // Occupancy: the 64x64 tile is held to 128 registers (4 waves per SIMD; chunk 1 spills a
 // few), the 64x128 tile to 168 (3 waves): both are latency-bound, and a 4096^2 output's
 // 4096 64x64 tiles then fill 1024 slots in exactly four rounds.
-template <int WMB, int WNB, int CHUNK, bool SUBN>
+template <int WMB, int WNB, int CHUNK, bool SUBN, bool DRC = false>
 __global__ __launch_bounds__(256, (WMB == 1 && WNB == 1) ? 4 : (WMB == 1 && WNB == 2) ? 3 : 1)
 void k_gemm_mfma(GemmArgs g) {
     __shared__ __align__(16) float smem[MfmaTile<WMB, WNB>::SMEM_FLOATS];
-    gemm_tile_mfma<WMB, WNB, CHUNK, SUBN>(g, (int)blockIdx.y, (int)blockIdx.x, (int)threadIdx.x,
-                                          smem);
+    gemm_tile_mfma<WMB, WNB, CHUNK, SUBN, false, DRC>(g, (int)blockIdx.y, (int)blockIdx.x,
+                                                      (int)threadIdx.x, smem);
 }
 
 using GemmKernel = void (*)(GemmArgs);
@@ -41,16 +41,26 @@ struct GemmConfig {
     int bm, bn;
     GemmKernel kernel[2];          // VALU tile, any chunk: [subn]
     GemmKernel mfma[6][2];         // wave tile: [log2(chunk)][subn], chunk in {1, 2, ..., 32}
+    // wave tile with directed-rounding chunk leaves, chunks 8-32: [log2(chunk) - 3] (nullptr
+    // where the tile has none), for the chunk sums of mantissa-only accumulators of 20 or more
+    // bits, which have no other fast leaf.  For narrower accumulators they measured no faster
+    // than the TwoSum round-to-odd leaves (8e7m operands) and 30 % slower on exact sums.
+    GemmKernel drc[3];
 };
 
 #define LOF_GEMM_CFG(BM, BN, TM, TN, BK)                                                    \
     {#BM "x" #BN "/" #TM "x" #TN "/k" #BK, BM, BN,                                           \
-     {k_gemm<BM, BN, TM, TN, BK, false>, k_gemm<BM, BN, TM, TN, BK, true>}, {}}
+     {k_gemm<BM, BN, TM, TN, BK, false>, k_gemm<BM, BN, TM, TN, BK, true>}, {}, {}}
 #define LOF_MFMA_PAIR(WMB, WNB, C) {k_gemm_mfma<WMB, WNB, C, false>, k_gemm_mfma<WMB, WNB, C, true>}
+#define LOF_MFMA_TABLE(WMB, WNB)                                                            \
+    {LOF_MFMA_PAIR(WMB, WNB, 1), LOF_MFMA_PAIR(WMB, WNB, 2), LOF_MFMA_PAIR(WMB, WNB, 4),      \
+     LOF_MFMA_PAIR(WMB, WNB, 8), LOF_MFMA_PAIR(WMB, WNB, 16), LOF_MFMA_PAIR(WMB, WNB, 32)}
 #define LOF_GEMM_MFMA_CFG(WMB, WNB)                                                         \
-    {"mfma" #WMB #WNB, 64 * WMB, 64 * WNB, {nullptr, nullptr},                               \
-     {LOF_MFMA_PAIR(WMB, WNB, 1), LOF_MFMA_PAIR(WMB, WNB, 2), LOF_MFMA_PAIR(WMB, WNB, 4),     \
-      LOF_MFMA_PAIR(WMB, WNB, 8), LOF_MFMA_PAIR(WMB, WNB, 16), LOF_MFMA_PAIR(WMB, WNB, 32)}}
+    {"mfma" #WMB #WNB, 64 * WMB, 64 * WNB, {nullptr, nullptr}, LOF_MFMA_TABLE(WMB, WNB), {}}
+#define LOF_GEMM_MFMA_DRC_CFG(WMB, WNB)                                                     \
+    {"mfma" #WMB #WNB, 64 * WMB, 64 * WNB, {nullptr, nullptr}, LOF_MFMA_TABLE(WMB, WNB),      \
+     {k_gemm_mfma<WMB, WNB, 8, false, true>, k_gemm_mfma<WMB, WNB, 16, false, true>,          \
+      k_gemm_mfma<WMB, WNB, 32, false, true>}}
 
 const GemmConfig kConfigs[] = {
     LOF_GEMM_CFG(64, 64, 4, 4, 16),
@@ -60,8 +70,8 @@ const GemmConfig kConfigs[] = {
     LOF_GEMM_CFG(32, 32, 2, 2, 32),
     LOF_GEMM_MFMA_CFG(2, 2),
     LOF_GEMM_MFMA_CFG(2, 1),
-    LOF_GEMM_MFMA_CFG(1, 1),
-    LOF_GEMM_MFMA_CFG(1, 2),
+    LOF_GEMM_MFMA_DRC_CFG(1, 1),
+    LOF_GEMM_MFMA_DRC_CFG(1, 2),
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kMfma64 = 7, kMfma64x128 = 8;
@@ -81,6 +91,7 @@ bool config_valid(int config, int chunk) {
 GemmKernel pick_kernel(const GemmConfig& c, const GemmArgs& g) {
     if (c.kernel[0]) return c.kernel[g.subn ? 1 : 0];
     const int l2 = 31 - __builtin_clz((unsigned)g.chunk);
+    if (l2 >= 3 && c.drc[0] && g.dr_kind == 1 && g.range_f == 0u) return c.drc[l2 - 3];
     return c.mfma[l2][g.subn ? 1 : 0];
 }
 
@@ -366,8 +377,11 @@ LOF_API lof_status lof_gemm_autotune(const lof_gemm_desc* desc, int32_t iters,
     }
     int best = -1;
     float best_t = 0.f;
+    // candidates: the wave tiles for the chunks they take (the VALU tiles, which keep the
+    // accumulator in double, measured 6-25x slower there), else the VALU tiles
+    const bool wave = chunk_fits_wave_tile(g.chunk);
     for (int c = 0; c < kNumConfigs && s == LOF_OK; ++c) {
-        if (!config_valid(c, g.chunk)) continue;
+        if (!config_valid(c, g.chunk) || is_mfma(c) != wave) continue;
         s = launch_gemm(g, c, st);  // warm-up
         if (s != LOF_OK) break;
         (void)hipEventRecord(e0, st);

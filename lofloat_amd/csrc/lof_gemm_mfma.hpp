@@ -250,6 +250,80 @@ __device__ __forceinline__ void leaves_dr16(const float (&a)[16], float b, float
     }
 }
 
+// RD / RU of na[o] - p[o] for 16 outputs, under the same mode switch as fma_down_up16.
+__device__ __forceinline__ void sub_down_up16(const float (&na)[16], const f32x16& p,
+                                              float (&rd)[16], float (&ru)[16]) {
+    asm volatile(
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 2\n\t"
+        "s_nop 1\n\t"
+        "v_sub_f32 %0, %32, %48\n\t"
+        "v_sub_f32 %1, %33, %49\n\t"
+        "v_sub_f32 %2, %34, %50\n\t"
+        "v_sub_f32 %3, %35, %51\n\t"
+        "v_sub_f32 %4, %36, %52\n\t"
+        "v_sub_f32 %5, %37, %53\n\t"
+        "v_sub_f32 %6, %38, %54\n\t"
+        "v_sub_f32 %7, %39, %55\n\t"
+        "v_sub_f32 %8, %40, %56\n\t"
+        "v_sub_f32 %9, %41, %57\n\t"
+        "v_sub_f32 %10, %42, %58\n\t"
+        "v_sub_f32 %11, %43, %59\n\t"
+        "v_sub_f32 %12, %44, %60\n\t"
+        "v_sub_f32 %13, %45, %61\n\t"
+        "v_sub_f32 %14, %46, %62\n\t"
+        "v_sub_f32 %15, %47, %63\n\t"
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 1\n\t"
+        "s_nop 1\n\t"
+        "v_sub_f32 %16, %32, %48\n\t"
+        "v_sub_f32 %17, %33, %49\n\t"
+        "v_sub_f32 %18, %34, %50\n\t"
+        "v_sub_f32 %19, %35, %51\n\t"
+        "v_sub_f32 %20, %36, %52\n\t"
+        "v_sub_f32 %21, %37, %53\n\t"
+        "v_sub_f32 %22, %38, %54\n\t"
+        "v_sub_f32 %23, %39, %55\n\t"
+        "v_sub_f32 %24, %40, %56\n\t"
+        "v_sub_f32 %25, %41, %57\n\t"
+        "v_sub_f32 %26, %42, %58\n\t"
+        "v_sub_f32 %27, %43, %59\n\t"
+        "v_sub_f32 %28, %44, %60\n\t"
+        "v_sub_f32 %29, %45, %61\n\t"
+        "v_sub_f32 %30, %46, %62\n\t"
+        "v_sub_f32 %31, %47, %63\n\t"
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 0\n\t"
+        "s_nop 1"
+        : "=&v"(rd[0]), "=&v"(rd[1]), "=&v"(rd[2]), "=&v"(rd[3]), "=&v"(rd[4]), "=&v"(rd[5]),
+          "=&v"(rd[6]), "=&v"(rd[7]), "=&v"(rd[8]), "=&v"(rd[9]), "=&v"(rd[10]), "=&v"(rd[11]),
+          "=&v"(rd[12]), "=&v"(rd[13]), "=&v"(rd[14]), "=&v"(rd[15]),
+          "=&v"(ru[0]), "=&v"(ru[1]), "=&v"(ru[2]), "=&v"(ru[3]), "=&v"(ru[4]), "=&v"(ru[5]),
+          "=&v"(ru[6]), "=&v"(ru[7]), "=&v"(ru[8]), "=&v"(ru[9]), "=&v"(ru[10]), "=&v"(ru[11]),
+          "=&v"(ru[12]), "=&v"(ru[13]), "=&v"(ru[14]), "=&v"(ru[15])
+        : "v"(na[0]), "v"(na[1]), "v"(na[2]), "v"(na[3]), "v"(na[4]), "v"(na[5]), "v"(na[6]),
+          "v"(na[7]), "v"(na[8]), "v"(na[9]), "v"(na[10]), "v"(na[11]), "v"(na[12]),
+          "v"(na[13]), "v"(na[14]), "v"(na[15]),
+          "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]),
+          "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]),
+          "v"(p[14]), "v"(p[15]));
+}
+
+// The chunk leaf on the negated accumulator: nacc[o] <- -round(RTO24(p[o] + acc[o])) from the
+// directed roundings of y = nacc - p = -(p + acc), for a mantissa-only accumulator.
+// Round-to-odd and both roundings are odd functions, so the construction of leaves_dr16
+// applied to y gives the negated result; only an exact zero changes: -(RTO zero of x) is
+// roundTowardPositive's zero of y, i.e. ru.
+__device__ __forceinline__ void leaves_dr16_neg(const f32x16& p, float (&nacc)[16],
+                                                const GemmArgs& g) {
+    float rd[16], ru[16];
+    sub_down_up16(nacc, p, rd, ru);
+#pragma unroll
+    for (int o = 0; o < 16; ++o) {
+        const uint32_t d = __float_as_uint(rd[o]), u = __float_as_uint(ru[o]);
+        const uint32_t t = rd[o] == ru[o] ? u : (min(d, u) | 1u);
+        nacc[o] = __uint_as_float((t + g.dr_half + __builtin_amdgcn_ubfe(t, g.dr_rf, g.dr_lsb)) &
+                                  g.dr_mask);
+    }
+}
+
 // Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
 // results are committed where valid, outputs needing the general path are recorded in a bit
 // mask, and one wave-uniform branch runs the general path for them (acc still holds their
@@ -329,7 +403,11 @@ __device__ __forceinline__ bool slice_bounds(uint32_t amax, uint32_t mnk, uint32
 // CONV: B is gathered from g.cv's activation and D / C are stored at its strided output
 // positions (implicit-GEMM convolution); a separate instantiation, so the plain GEMM
 // kernels carry none of its registers.
-template <int WMB, int WNB, int CHUNK, bool SUBN, bool CONV = false>
+// DRC: chunk sums (chunks 8-32) take the directed-rounding leaves in place of the TwoSum
+// round-to-odd leaves: the fast leaf for mantissa-only accumulators of 20 or more bits, which
+// no other leaf covers.  An instantiation of its own, since both kinds in one kernel spill
+// and slow the exact-sum workloads by a fifth.
+template <int WMB, int WNB, int CHUNK, bool SUBN, bool CONV = false, bool DRC = false>
 __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, int tile_n,
                                                int tid, float* smem) {
     using T = MfmaTile<WMB, WNB>;
@@ -343,6 +421,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     constexpr bool kKey = CHUNK <= 4;
     // exact chunk sums on the bf16 MFMAs (chunks 8-32) or the f32 MFMA (chunks 2, 4)
     constexpr bool kBf16 = CHUNK >= 8;
+    // directed-rounding leaves at chunk 1, and for the sums of chunks 8-32 in the DRC
+    // instantiation; otherwise chunk sums keep the TwoSum round-to-odd leaves
+    constexpr bool kDr = CHUNK == 1 || (DRC && CHUNK >= 8);
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -675,7 +756,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         int hs = 0;
         const bool try_fast = sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
                               (g.sum_off & 3) != 3;
-        const bool try_dr = CHUNK == 1 && sfin && g.dr_kind != 0 && !(g.sum_off & 16);
+        // chunk sums: mantissa-only accumulators only (the DRC kernels' accumulators)
+        const bool try_dr = kDr && sfin && (CHUNK == 1 ? g.dr_kind != 0 : g.dr_kind == 1) &&
+                            !(g.sum_off & 16);
         if (try_fast || try_dr) {
             uint32_t amax = 0u, mnk = 0xFFFFFFFFu;
 #pragma unroll
@@ -725,9 +808,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     hs = 0;
                 }
               }
-              dr_ok = try_dr && !sum_ok && !pk_ok && amax <= 0x7F800000u &&
-                      (g.dr_kind == 1 || S <= g.sum_ehi);
             }
+            dr_ok = try_dr && !sum_ok && !pk_ok && amax <= 0x7F800000u &&
+                    (g.dr_kind == 1 || S <= g.sum_ehi);
         }
 
         const int kmax = min(BK, g.K - k0);
@@ -945,7 +1028,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             }
           }
         } else {
-        set_form(sum_ok || rto_ok);
+        // chunk sums: exact-sum leaves, else the directed-rounding leaves (DRC) or the TwoSum
+        // round-to-odd leaves, else the general leaves
+        set_form(sum_ok || dr_ok || (!kDr && rto_ok));
         float (&acc)[WMB][WNB][16] = nacc;
 #pragma unroll
         for (int q = 0; q < BK / CHUNK; ++q) {
@@ -1092,7 +1177,12 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                             for (int r = 0; r < 16; ++r)
                                 nacc[bi][bj][r] = leaf_exact<false>(nacc[bi][bj][r] - part[bi][bj][r], g);
                 }
-            } else if (rto_ok) {   // nacc <- -round(RTO24(p + acc)) for any in-range sum
+            } else if (kDr && dr_ok) {   // nacc <- -round(RTO24(p + acc)), directed roundings
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj) leaves_dr16_neg(part[bi][bj], nacc[bi][bj], g);
+            } else if (!kDr && rto_ok) {   // nacc <- -round(RTO24(p + acc)) for any in-range sum
                 if (g.sum_f16) {
 #pragma unroll
                     for (int bi = 0; bi < WMB; ++bi)

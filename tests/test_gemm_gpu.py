@@ -101,7 +101,7 @@ def test_shapes_all_configs(dev, shape):
                         f"{shape} {kind} chunk {chunk} config {cfg}")
 
 
-MFMA_CONFIGS = (5, 6, 7, 8)
+MFMA_CONFIGS = (5, 6, 7, 8)   # wave tiles 128x128, 128x64, 64x64, 64x128
 
 
 @pytest.mark.parametrize("kind", ["e4m3_small", "mixed", "normal", "special", "e4m3", "tiny"])
@@ -394,15 +394,17 @@ def test_rto_leaves(dev, chunk):
                             f"rto chunk {chunk} config {cfg} bits {bits:#x}")
 
 
+@pytest.mark.parametrize("chunk", [1, 8, 32])
 @pytest.mark.parametrize("kind", ["uniform", "mx", "wide", "overflow", "tiny"])
-def test_directed_rounding_leaves(dev, kind):
-    """Chunk 1 with products that FP32 cannot hold (24-bit significands): the directed-rounding
-    leaves (round-to-odd from the FMA rounded down and up) for mantissa-only accumulators of
-    every width and for binary16-shaped formats, against the oracle on every wave tile, and
-    the same tiles with those leaves off (0x8000).  Kinds: FP32 uniform(-1, 1) operands (the
-    reference harness's lof_gemm inputs), MX-rounded e4m3 operands whose overflowing
-    elements pass through unrounded, magnitudes across the FP32 range, sums that overflow to
-    FLT_MAX | 1 and inf, and products in the FP32 subnormal range."""
+def test_directed_rounding_leaves(dev, kind, chunk):
+    """Products or chunk sums that FP32 cannot hold exactly (24-bit significands): the
+    directed-rounding leaves (round-to-odd from the sum rounded down and up) for
+    mantissa-only accumulators of every width and for binary16-shaped formats, at chunk 1 and
+    on the chunk sums of chunks 8 and 32, against the oracle on every wave tile, and the same
+    tiles with those leaves off (0x8000).  Kinds: FP32 uniform(-1, 1) operands (the reference
+    harness's lof_gemm inputs), MX-rounded e4m3 operands whose overflowing elements pass
+    through unrounded, magnitudes across the FP32 range, sums that overflow to FLT_MAX | 1
+    and inf, and products in the FP32 subnormal range."""
     import lofloat_amd as lof
     g = np.random.default_rng({"uniform": 1, "mx": 2, "wide": 3, "overflow": 4, "tiny": 5}[kind])
     M, N, K = 80, 96, 192
@@ -423,11 +425,11 @@ def test_directed_rounding_leaves(dev, kind):
     else:
         A = (g.standard_normal((M, K)) * 2.0 ** -70).astype(np.float32)
         B = (g.standard_normal((K, N)) * 2.0 ** -62).astype(np.float32)
-    accs = [_accums(lof, m) for m in (0, 5, 10, 19, 20, 21, 22, 23)]
-    accs += [_accums(lof, fmt=FP16_SAT), _accums(lof, fmt=FP16_EXT)]
+    accs = [_accums(lof, m, chunk) for m in (0, 5, 10, 19, 20, 21, 22, 23)]
+    accs += [_accums(lof, fmt=FP16_SAT, chunk=chunk), _accums(lof, fmt=FP16_EXT, chunk=chunk)]
     for acc, oacc in accs:
         ref = _ref(A, B, oacc)
-        what = f"{kind} m={acc.mant_bits if acc.fmt is None else 'fp16'}"
+        what = f"{kind} chunk {chunk} m={acc.mant_bits if acc.fmt is None else 'fp16'}"
         for cfg in MFMA_CONFIGS:
             for bits in (0, 0x8000):
                 assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
