@@ -349,3 +349,129 @@ def test_packed_binary16_rto_leaves():
         assert _bits(got) == _bits(ref), (a, b, acc, got, ref)
         checked += 1
     assert checked > 1200 and inexact > 120, (checked, inexact)
+
+
+def _directed(x, down, zero_neg):
+    """x (an exact rational) rounded to FP32 toward -inf (down) or +inf, subnormals and
+    overflow included; zero_neg: the sign an exact zero takes under that rounding."""
+    from fractions import Fraction
+    fmax = Fraction(float(np.finfo(np.float32).max))
+    if x == 0:
+        return F32(-0.0) if zero_neg else F32(0.0)
+    if x > fmax:
+        return F32(fmax) if down else F32(np.inf)
+    if x < -fmax:
+        return F32(-np.inf) if down else F32(-fmax)
+    c = F32(float(x))
+    inf = F32(np.inf)
+    if down:
+        while Fraction(float(c)) > x:
+            c = np.nextafter(c, -inf)
+        while Fraction(float(np.nextafter(c, inf))) <= x:
+            c = np.nextafter(c, inf)
+        if c == 0:                       # a tiny x > 0 rounds down to +0
+            c = F32(0.0)
+    else:
+        while Fraction(float(c)) < x:
+            c = np.nextafter(c, inf)
+        while Fraction(float(np.nextafter(c, -inf))) >= x:
+            c = np.nextafter(c, -inf)
+        if c == 0:                       # a tiny x < 0 rounds up to -0
+            c = F32(-0.0)
+    return F32(c)
+
+
+def _zero_sign(t1, t2, down):
+    """IEEE sign of an exact zero t1 + t2 under a directed rounding: two zeros of one sign keep
+    it, otherwise -0 rounding down and +0 rounding up."""
+    neg = lambda v: math.copysign(1.0, float(v)) < 0
+    if float(t1) == 0 and float(t2) == 0 and neg(t1) == neg(t2):
+        return neg(t1)
+    return down
+
+
+def _dr_round(t, m, fmt):
+    """The leaf's round of the RTO pattern t: integer RNE at bit 23 - m with the kernel's
+    constants (mantissa-only), or the binary16 conversion (binary16-shaped)."""
+    if fmt is not None:
+        return F32(np.float16(_float(t)))
+    rf = 23 - m
+    half = (1 << (rf - 1)) - 1 if rf > 0 else 0
+    lsb = (t >> rf) & 1 if rf > 0 else 0
+    return _float((t + half + lsb) & ~((1 << rf) - 1) & 0xFFFFFFFF)
+
+
+def test_directed_rounding_leaves():
+    """The directed-rounding leaves: from d = RD(x) and u = RU(x) of the exact sum (computed
+    here in rational arithmetic), RTO24(x) is d when d == u, else min(d, u) | 1 on the bit
+    patterns; it must equal the oracle's nextafter-form round-to-odd FMA (pinned by exact
+    rationals), and its round must equal the reference leaf.  Chunk 1 (x = a b + acc, the
+    plain form) and chunk sums (y = nacc - p = -(p + acc), the negated form, which takes u for
+    an exact zero) over FP32 uniform and wide operands, overflow, FP32-subnormal products,
+    exact cancellations and signed zeros, mantissa-only accumulators of 0-23 bits and the
+    binary16-shaped format below 2^15."""
+    from fractions import Fraction
+    g = np.random.default_rng(919)
+    fmax = float(np.finfo(np.float32).max)
+    checked = odd = 0
+    for it in range(3000):
+        kind = it % 6
+        m = int(g.integers(0, 24))
+        fmt = None
+        if kind == 0:
+            a, b = (F32(g.uniform(-1, 1)) for _ in range(2))
+            acc = F32(g.uniform(-4, 4))
+        elif kind == 1:
+            a, b = (F32(g.standard_normal() * 2.0 ** g.integers(-60, 60)) for _ in range(2))
+            acc = F32(g.standard_normal() * 2.0 ** g.integers(-100, 100))
+        elif kind == 2:                  # sums near and past FLT_MAX
+            a = F32(g.uniform(1, 2) * 2.0 ** 63 * g.choice([-1, 1]))
+            b = F32(g.uniform(1, 2) * 2.0 ** 64)
+            acc = F32(g.choice([-1, 1]) * fmax * g.uniform(0.5, 1))
+        elif kind == 3:                  # products in the FP32 subnormal range
+            a = F32(g.standard_normal() * 2.0 ** -70)
+            b = F32(g.standard_normal() * 2.0 ** -62)
+            acc = F32(g.standard_normal() * 2.0 ** -140)
+        elif kind == 4:                  # exact cancellations and signed zeros
+            a, b = (F32(g.uniform(-1, 1)) for _ in range(2))
+            acc = F32(-(np.float64(a) * np.float64(b))) if g.random() < 0.5 else F32(0.0)
+            if g.random() < 0.5:
+                a = F32(0.0)
+            a, acc = (F32(-v) if g.random() < 0.5 else v for v in (a, acc))
+        else:                            # binary16-shaped accumulator, sums below 2^15
+            fmt = (16, 10, 15, int(g.integers(0, 2)), 0, 0)
+            a, b = (F32(g.standard_normal() * 2.0 ** g.integers(-12, 4)) for _ in range(2))
+            acc = F32(oracle.virtual_round(np.array([g.standard_normal() * 2.0 ** 8], F32),
+                                           fmt)[0])
+        if fmt is None:
+            acc = F32(oracle.virtual_round_mant_scalar(float(acc), m))
+        if not np.isfinite(acc):         # a start value near FLT_MAX rounded up to inf
+            continue
+        pm = Fraction(float(a)) * Fraction(float(b))
+        x = pm + Fraction(float(acc))
+        if fmt is not None and abs(x) >= 2 ** 15:
+            continue
+        neg_p = (math.copysign(1.0, float(a)) < 0) != (math.copysign(1.0, float(b)) < 0)
+        p0 = F32(-0.0) if neg_p else F32(0.0)    # the product's zero, for the zero sign
+        # chunk 1, plain form
+        d = _directed(x, True, _zero_sign(p0 if pm == 0 else 1, acc, True))
+        u = _directed(x, False, _zero_sign(p0 if pm == 0 else 1, acc, False))
+        t = _bits(d) if d == u else (min(_bits(d), _bits(u)) | 1)
+        rto = oracle.rto_fma(float(a), float(b), float(acc))
+        assert t == _bits(rto), (a, b, acc, _float(t), rto)
+        odd += d != u
+        ref = F32(oracle.virtual_round_mant_scalar(rto, m) if fmt is None
+                  else oracle.virtual_round_scalar(rto, fmt))
+        assert _bits(_dr_round(t, m, fmt)) == _bits(ref), (a, b, acc, m, fmt)
+        # chunk sum, negated form: p is an FP32 chunk sum, y = (-acc) - p
+        if fmt is None and abs(float(a) * float(b)) <= fmax / 2:
+            p = F32(np.float64(a) * np.float64(b))
+            nacc = F32(-acc)
+            y = Fraction(float(nacc)) - Fraction(float(p))
+            dn = _directed(y, True, _zero_sign(nacc, F32(-p), True))
+            un = _directed(y, False, _zero_sign(nacc, F32(-p), False))
+            tn = _bits(un) if dn == un else (min(_bits(dn), _bits(un)) | 1)
+            ref_n = _ref_leaf(p, acc, None, m)
+            assert _bits(F32(-_dr_round(tn, m, None))) == _bits(ref_n), (p, acc, m)
+        checked += 1
+    assert checked > 2800 and odd > 1000, (checked, odd)

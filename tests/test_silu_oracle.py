@@ -10,7 +10,12 @@ Lof_kernel.cu:240-258) by what the mathematics fixes -- the reference ships no t
 * against the reference layer's torch op chain (PWLSiLU._forward_ref: clamp, index,
   torch.lerp, relu where |x| > R): within twice the chord's rounding budget inside
   [-R, R], equal outside;
-* the tables PWLSiLU builds approximate SiLU to the chord bound of each method.
+* the tables PWLSiLU builds approximate SiLU to the chord bound of each method;
+* bit for bit, the kernel's float operation sequence (Lof_kernel.cu:248-254, 261) evaluated
+  in exact rational arithmetic with one round-to-nearest-even to float per operation:
+  inv_step = N / (2R), u = (xc + R) * inv_step, frac = u - i, y = fma(frac, y1 - y0, y0)
+  (one rounding); the same data shows that mul-then-add, x inv_step + R inv_step, and
+  double-precision intermediates each give different bits, so none of them passes.
 """
 import numpy as np
 import pytest
@@ -94,3 +99,55 @@ def test_tables_approximate_silu(method, bound):
     silu = x / (1 + np.exp(-x.astype(np.float64)))
     err = np.abs(y - silu).max()
     assert err < bound, err
+
+
+def _rn32(x):
+    """Round-to-nearest-even of an exact rational to float32 (no overflow in this use)."""
+    from fractions import Fraction
+    if x == 0:
+        return np.float32(0.0)
+    c = np.float32(float(x))
+    inf = np.float32(np.inf)
+    while Fraction(float(c)) > x:
+        c = np.nextafter(c, -inf)
+    while Fraction(float(np.nextafter(c, inf))) <= x:
+        c = np.nextafter(c, inf)
+    lo, hi = c, np.nextafter(c, inf)          # lo <= x < hi
+    dl, dh = x - Fraction(float(lo)), Fraction(float(hi)) - x
+    if dl < dh or (dl == dh and int(np.array([lo]).view(np.uint32)[0]) % 2 == 0):
+        return np.float32(lo)
+    return np.float32(hi)
+
+
+@pytest.mark.parametrize("N,R", [(16, 6.0), (64, 3.7), (256, 8.0), (1024, 5.3)])
+def test_op_sequence_exact_rationals(N, R):
+    from fractions import Fraction as Q
+    g = np.random.default_rng(N + 7)
+    lut = (g.standard_normal(N + 1) * 3).astype(np.float32)
+    Rf = np.float32(R)
+    x = np.concatenate([g.uniform(-R, R, 3000), g.uniform(-R, R, 64) * 1e-6]).astype(np.float32)
+    got = oracle.pwl_silu(x, lut, R)
+    inv = _rn32(Q(N) / Q(float(_rn32(2 * Q(float(Rf))))))
+    alt = {"mul_add": 0, "split_scale": 0, "double": 0}
+    for k, xv in enumerate(x):
+        xc = min(max(float(xv), -float(Rf)), float(Rf))
+        u = _rn32(Q(float(_rn32(Q(xc) + Q(float(Rf))))) * Q(float(inv)))
+        i = min(int(u), N - 1)
+        frac = _rn32(Q(float(u)) - i)
+        y0, y1 = Q(float(lut[i])), Q(float(lut[i + 1]))
+        d = _rn32(y1 - y0)
+        y = _rn32(Q(float(frac)) * Q(float(d)) + y0)
+        assert got[k].view(np.uint32) == y.view(np.uint32), (xv, got[k], y)
+        # the plausible slips, on the same element
+        alt["mul_add"] += _rn32(Q(float(_rn32(Q(float(frac)) * Q(float(d))))) + y0) != y
+        u2 = _rn32(Q(float(_rn32(Q(xc) * Q(float(inv))))) +
+                   Q(float(_rn32(Q(float(Rf)) * Q(float(inv))))))
+        i2 = min(int(u2), N - 1)
+        if i2 == i:
+            alt["split_scale"] += _rn32(Q(float(u2)) - i) != frac
+        yd = (float(frac) * (float(lut[i + 1]) - float(lut[i]))) + float(lut[i])
+        alt["double"] += np.float32(yd) != y
+    # with a power-of-two inv_step both scalings are exact, so they cannot differ
+    if np.frexp(inv)[0] == 0.5:
+        alt.pop("split_scale")
+    assert all(v > 0 for v in alt.values()), alt
