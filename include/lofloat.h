@@ -187,7 +187,11 @@ typedef struct lof_gemm_desc {
  *           the 64x64 and 64x128 wave tiles for the sums of chunks 8 to 32 into
  *           mantissa-only accumulators of 20 or more bits);
  *   0x10000 turns off the bf16 MFMA chunk sums (exact chunks of 8 to 32 then sum on the
- *           VALU chain; chunks of 2 and 4 always use the f32 MFMA).
+ *           VALU chain; chunks of 2 and 4 always use the f32 MFMA);
+ *   0x20000 turns off the mixed-precision FMA leaves of binary16-shaped accumulators
+ *           (v_fma_mixlo_f16, one rounding of the exact sum: at chunk 1 on lanes whose
+ *           products or sums are inexact in FP32, and for every chunk sum of a slice that
+ *           fails the exact-sum test).
  * Chunk sums run on MFMA only where every partial sum of the chunk is exact in FP32 (chunks of
  * 8 to 32 on the bf16 MFMAs, where the slice's values are all bfloat16 values; chunks of 2
  * and 4 on the f32 MFMA), and the

@@ -20,9 +20,10 @@ namespace lofamd {
 
 namespace {
 
-// the 64x64 tile at 4 waves per SIMD, as the GEMM's
+// the 64x64 tile at 4 waves per SIMD, as the GEMM's; the 128x64 tile held to 2 (256
+// registers: at 260 it drops to one wave per SIMD and runs 1.5x longer)
 template <int WMB, int WNB, int CHUNK, bool SUBN>
-__global__ __launch_bounds__(256, WMB == 1 ? 4 : 1)
+__global__ __launch_bounds__(256, WMB == 1 ? 4 : 2)
 void k_conv_mfma(GemmArgs g) {
     __shared__ __align__(16) float smem[MfmaTile<WMB, WNB>::SMEM_FLOATS];
     gemm_tile_mfma<WMB, WNB, CHUNK, SUBN, true>(g, (int)blockIdx.y, (int)blockIdx.x,
@@ -132,8 +133,9 @@ LOF_API lof_status lof_conv2d_f32(const lof_conv2d_desc* desc, int32_t config,
                          what, cfg, kCfgMfma11, kCfgMfma21);
     }
     // testing bits as in lof_gemm_f32: 0x100 general leaf path, 0x200 every chunk on VALU,
-    // 0x800 / 0x1000 / 0x2000 / 0x4000 / 0x8000 the exact-sum / RTO / binary16-held /
-    // packed binary16 / directed-rounding leaves off, 0x10000 the bf16 MFMA chunk sums off
+    // 0x800 / 0x1000 / 0x2000 / 0x4000 / 0x8000 / 0x20000 the exact-sum / RTO /
+    // binary16-held / packed binary16 / directed-rounding / mixed-precision FMA leaves off,
+    // 0x10000 the bf16 MFMA chunk sums off
     if (config >= 0) {
         if (config & 0x800) g.sum_off |= 1;
         if (config & 0x1000) g.sum_off |= 2;
@@ -141,6 +143,7 @@ LOF_API lof_status lof_conv2d_f32(const lof_conv2d_desc* desc, int32_t config,
         if (config & 0x4000) g.sum_off |= 8;
         if (config & 0x8000) g.sum_off |= 16;
         if (config & 0x10000) g.mfma_bf16 = 0;
+        if (config & 0x20000) g.sum_off |= 32;
         if (config & 0x100) {
             set_fast_path(&g, false, 0);
             g.dr_kind = 0;

@@ -326,6 +326,7 @@ LOF_API lof_status lof_gemm_f32(const lof_gemm_desc* desc, int32_t config, lof_s
     if (config >= 0 && (config & 0x4000)) g.sum_off |= 8;
     if (config >= 0 && (config & 0x8000)) g.sum_off |= 16;
     if (config >= 0 && (config & 0x10000)) g.mfma_bf16 = 0;
+    if (config >= 0 && (config & 0x20000)) g.sum_off |= 32;
     if (config >= 0 && (config & 0x100)) {
         set_fast_path(&g, false, 0);
         g.dr_kind = 0;

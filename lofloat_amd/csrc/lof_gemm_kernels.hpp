@@ -113,7 +113,8 @@ struct GemmArgs {
     int32_t sum_h10;     // mantissa-only with m = 10: binary16's significand, any exponent
     int32_t sum_off;     // testing: bit 0 exact-sum leaves off, bit 1 RTO leaves off,
                          // bit 2 binary16-held chunk-1 leaves off, bit 3 their packed form,
-                         // bit 4 the directed-rounding chunk-1 leaves
+                         // bit 4 the directed-rounding leaves, bit 5 the mixed-precision
+                         // FMA leaves of binary16-shaped accumulators
     // directed-rounding chunk-1 leaves (any product, exact or not): 0 off, 1 mantissa-only
     // accumulator (round-to-nearest-even at bit dr_rf of the pattern), 2 binary16-shaped
     int32_t dr_kind;

@@ -160,6 +160,117 @@ __device__ __forceinline__ float leaf_rto(float p, float na, const GemmArgs& g) 
     return leaf_exact<F16>(__uint_as_float(e == 0.0f ? tb : odd), g);
 }
 
+// binary16(h - a b) for FP32 a, b and a binary16 h: v_fma_mixlo_f16 rounds the exact result
+// once (the product is not rounded first; FP32-subnormal products and binary16 subnormals
+// included; an exact zero takes round-to-nearest's sign).  For a binary16-shaped accumulator
+// whose sums stay below 2^15 this is the reference leaf round(RTO24(a b + acc)) on the negated
+// accumulator h = -acc: round-to-odd keeps 13 more bits than binary16, so rounding it again
+// gives the single rounding of the exact sum.
+__device__ __forceinline__ _Float16 mix_fnma16(float a, float b, _Float16 h) {
+    _Float16 r;
+    asm("v_fma_mixlo_f16 %0, -%1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(r) : "v"(a), "v"(b), "v"(h));
+    return r;
+}
+
+// na[o] <- binary16(na[o] - p[o]) for FP32 na and p, one rounding of the exact difference,
+// for 16 outputs in place: the chunk leaf of a binary16-shaped accumulator (na = -acc holds a
+// binary16 value) on the same argument.  One asm block with tied operands, so the leaves
+// add no registers.
+__device__ __forceinline__ void mix_sub16x16(const f32x16& p, float (&na)[16]) {
+    asm(
+        "v_fma_mixlo_f16 %0, %16, -1.0, %0 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %1, %17, -1.0, %1 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %2, %18, -1.0, %2 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %3, %19, -1.0, %3 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %4, %20, -1.0, %4 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %5, %21, -1.0, %5 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %6, %22, -1.0, %6 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %7, %23, -1.0, %7 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %8, %24, -1.0, %8 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %9, %25, -1.0, %9 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %10, %26, -1.0, %10 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %11, %27, -1.0, %11 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %12, %28, -1.0, %12 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %13, %29, -1.0, %13 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %14, %30, -1.0, %14 op_sel_hi:[0,0,0]\n\t"
+        "v_fma_mixlo_f16 %15, %31, -1.0, %15 op_sel_hi:[0,0,0]\n\t"
+        "v_cvt_f32_f16 %0, %0\n\t"
+        "v_cvt_f32_f16 %1, %1\n\t"
+        "v_cvt_f32_f16 %2, %2\n\t"
+        "v_cvt_f32_f16 %3, %3\n\t"
+        "v_cvt_f32_f16 %4, %4\n\t"
+        "v_cvt_f32_f16 %5, %5\n\t"
+        "v_cvt_f32_f16 %6, %6\n\t"
+        "v_cvt_f32_f16 %7, %7\n\t"
+        "v_cvt_f32_f16 %8, %8\n\t"
+        "v_cvt_f32_f16 %9, %9\n\t"
+        "v_cvt_f32_f16 %10, %10\n\t"
+        "v_cvt_f32_f16 %11, %11\n\t"
+        "v_cvt_f32_f16 %12, %12\n\t"
+        "v_cvt_f32_f16 %13, %13\n\t"
+        "v_cvt_f32_f16 %14, %14\n\t"
+        "v_cvt_f32_f16 %15, %15"
+        : "+v"(na[0]), "+v"(na[1]), "+v"(na[2]), "+v"(na[3]), "+v"(na[4]), "+v"(na[5]), "+v"(na[6]), "+v"(na[7]), "+v"(na[8]), "+v"(na[9]), "+v"(na[10]), "+v"(na[11]), "+v"(na[12]), "+v"(na[13]), "+v"(na[14]), "+v"(na[15])
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]), "v"(p[14]), "v"(p[15]));
+}
+
+// na[o] <- binary16(na[o] - p[o]) when every difference is exact in FP32 (the exact-sum leaf
+// of a binary16-shaped accumulator: one subtraction and the conversion pair, all full rate),
+// for 16 outputs in place, tied like mix_sub16x16.
+__device__ __forceinline__ void exact_sub16x16(const f32x16& p, float (&na)[16]) {
+    asm(
+        "v_sub_f32 %0, %0, %16\n\t"
+        "v_sub_f32 %1, %1, %17\n\t"
+        "v_sub_f32 %2, %2, %18\n\t"
+        "v_sub_f32 %3, %3, %19\n\t"
+        "v_sub_f32 %4, %4, %20\n\t"
+        "v_sub_f32 %5, %5, %21\n\t"
+        "v_sub_f32 %6, %6, %22\n\t"
+        "v_sub_f32 %7, %7, %23\n\t"
+        "v_sub_f32 %8, %8, %24\n\t"
+        "v_sub_f32 %9, %9, %25\n\t"
+        "v_sub_f32 %10, %10, %26\n\t"
+        "v_sub_f32 %11, %11, %27\n\t"
+        "v_sub_f32 %12, %12, %28\n\t"
+        "v_sub_f32 %13, %13, %29\n\t"
+        "v_sub_f32 %14, %14, %30\n\t"
+        "v_sub_f32 %15, %15, %31\n\t"
+        "v_cvt_f16_f32 %0, %0\n\t"
+        "v_cvt_f16_f32 %1, %1\n\t"
+        "v_cvt_f16_f32 %2, %2\n\t"
+        "v_cvt_f16_f32 %3, %3\n\t"
+        "v_cvt_f16_f32 %4, %4\n\t"
+        "v_cvt_f16_f32 %5, %5\n\t"
+        "v_cvt_f16_f32 %6, %6\n\t"
+        "v_cvt_f16_f32 %7, %7\n\t"
+        "v_cvt_f16_f32 %8, %8\n\t"
+        "v_cvt_f16_f32 %9, %9\n\t"
+        "v_cvt_f16_f32 %10, %10\n\t"
+        "v_cvt_f16_f32 %11, %11\n\t"
+        "v_cvt_f16_f32 %12, %12\n\t"
+        "v_cvt_f16_f32 %13, %13\n\t"
+        "v_cvt_f16_f32 %14, %14\n\t"
+        "v_cvt_f16_f32 %15, %15\n\t"
+        "v_cvt_f32_f16 %0, %0\n\t"
+        "v_cvt_f32_f16 %1, %1\n\t"
+        "v_cvt_f32_f16 %2, %2\n\t"
+        "v_cvt_f32_f16 %3, %3\n\t"
+        "v_cvt_f32_f16 %4, %4\n\t"
+        "v_cvt_f32_f16 %5, %5\n\t"
+        "v_cvt_f32_f16 %6, %6\n\t"
+        "v_cvt_f32_f16 %7, %7\n\t"
+        "v_cvt_f32_f16 %8, %8\n\t"
+        "v_cvt_f32_f16 %9, %9\n\t"
+        "v_cvt_f32_f16 %10, %10\n\t"
+        "v_cvt_f32_f16 %11, %11\n\t"
+        "v_cvt_f32_f16 %12, %12\n\t"
+        "v_cvt_f32_f16 %13, %13\n\t"
+        "v_cvt_f32_f16 %14, %14\n\t"
+        "v_cvt_f32_f16 %15, %15"
+        : "+v"(na[0]), "+v"(na[1]), "+v"(na[2]), "+v"(na[3]), "+v"(na[4]), "+v"(na[5]), "+v"(na[6]), "+v"(na[7]), "+v"(na[8]), "+v"(na[9]), "+v"(na[10]), "+v"(na[11]), "+v"(na[12]), "+v"(na[13]), "+v"(na[14]), "+v"(na[15])
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7]), "v"(p[8]), "v"(p[9]), "v"(p[10]), "v"(p[11]), "v"(p[12]), "v"(p[13]), "v"(p[14]), "v"(p[15]));
+}
+
 // RD / RU: a[o] * b + c[o] rounded down and up, for 16 outputs.  The wave's FP32 rounding
 // mode (MODE.FP_ROUND[1:0]: 2 toward -inf, 1 toward +inf, 0 nearest even) is switched only
 // inside this one asm block, which ends in round-to-nearest-even again, so no FP operation
@@ -424,6 +535,10 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // directed-rounding leaves at chunk 1, and for the sums of chunks 8-32 in the DRC
     // instantiation; otherwise chunk sums keep the TwoSum round-to-odd leaves
     constexpr bool kDr = CHUNK == 1 || (DRC && CHUNK >= 8);
+    // binary16-shaped accumulators run the SUBN kernels (their subnormal range lies above
+    // FP32's), so the other kernels leave out their chunk-sum leaves -- except at chunk 32,
+    // whose kernel measured 129 registers of spills without them (a different schedule)
+    constexpr bool kF16Leaves = (SUBN || CHUNK >= 32) && !DRC;
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -754,6 +869,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // accumulators hold a NaN keeps the general leaf.
         bool sum_ok = false, rto_ok = false, h_ok = false, pk_ok = false, dr_ok = false;
         int hs = 0;
+        const bool f16 = g.sum_f16;
         const bool try_fast = sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
                               (g.sum_off & 3) != 3;
         // chunk sums: mantissa-only accumulators only (the DRC kernels' accumulators)
@@ -785,12 +901,15 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             if (try_fast) {
             sum_ok = exact && sL >= g.sum_lreq && S <= min(sL + 24, g.sum_ehi) &&
                      sTop <= Lf + 23 && Lf >= -126 && !(g.sum_off & 1);
-            rto_ok = !sum_ok && g.sum_rto && S <= g.sum_ehi && !(g.sum_off & 2);
+            // (chunk sums into a binary16-shaped accumulator take mix_sub16x16 for this leaf;
+            // with those off, the general leaves)
+            rto_ok = !sum_ok && g.sum_rto && S <= g.sum_ehi && !(g.sum_off & 2) &&
+                     !(CHUNK >= 8 && !CONV && f16 && (g.sum_off & 32));
             }
             if constexpr (CHUNK == 1) {
               if (try_fast) {
                 if (sum_ok && !(g.sum_off & 4)) {
-                    if (g.sum_f16) {
+                    if (f16) {
                         h_ok = true;
                     } else if (g.sum_h10) {
                         hs = 15 - S;
@@ -802,7 +921,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 // binary16-shaped format on RTO lanes (sums below 2^15, maybe inexact in
                 // FP32): the binary16 FMA rounds the exact sum once, which is what rounding
                 // the reference's RTO24 sum to 11 significant bits gives
-                if (rto_ok && g.sum_f16 && a16 && bW <= 11 && bLsb >= -24 && bTop <= 15 &&
+                if (rto_ok && f16 && a16 && bW <= 11 && bLsb >= -24 && bTop <= 15 &&
                     !(g.sum_off & 12)) {
                     pk_ok = true;
                     hs = 0;
@@ -811,11 +930,21 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             }
             dr_ok = try_dr && !sum_ok && !pk_ok && amax <= 0x7F800000u &&
                     (g.dr_kind == 1 || S <= g.sum_ehi);
+            // chunk 1, binary16-shaped accumulator, every sum below 2^15: the held leaves with
+            // the mixed FMA (mix_fnma16) round the exact a b + h once, so they take the lanes
+            // whose products or sums are inexact in FP32 as well
+            if constexpr (CHUNK == 1) {
+                if (dr_ok && g.dr_kind == 2 && !(g.sum_off & 32)) {
+                    h_ok = true;
+                    hs = 0;
+                    dr_ok = false;
+                }
+            }
         }
 
         const int kmax = min(BK, g.K - k0);
         if constexpr (CHUNK == 1) {
-          set_form(sum_ok || pk_ok || (rto_ok && !dr_ok));
+          set_form(sum_ok || pk_ok || h_ok || (rto_ok && !dr_ok));
           // nacc <- -round(a * b + acc): exact sums (fma(-a, b, nacc) is -(a * b + acc)
           // exactly) or RTO leaves on the exact product a * b
           auto leaves_c1 = [&](auto f16, auto rto) {
@@ -905,9 +1034,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     }
           };
           if (pk_ok) {
-            if (g.sum_f16) pk_leaves(std::false_type{});
+            if (f16) pk_leaves(std::false_type{});
             else pk_leaves(std::true_type{});
-          } else if (h_ok) {   // h <- binary16(fma(-a, b 2^hs, h)), h = nacc 2^hs (exact both ways)
+          } else if (h_ok) {   // h <- binary16(h - a b 2^hs), h = nacc 2^hs (exact both ways)
             _Float16 h[WMB][WNB][16];
 #pragma unroll
             for (int bi = 0; bi < WMB; ++bi)
@@ -938,7 +1067,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                                                  a4[bi][rq].w};
 #pragma unroll
                             for (int x = 0; x < 4; ++x)
-                                hv[x] = (_Float16)__builtin_fmaf(-av[x], b[bj], (float)hv[x]);
+                                hv[x] = mix_fnma16(av[x], b[bj], hv[x]);
                         }
             }
 #pragma unroll
@@ -949,7 +1078,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     for (int r = 0; r < 16; ++r)
                         nacc[bi][bj][r] = __builtin_ldexpf((float)h[bi][bj][r], -hs);
           } else if (sum_ok) {
-            if (g.sum_f16) leaves_c1(std::true_type{}, std::false_type{});
+            if (f16) leaves_c1(std::true_type{}, std::false_type{});
             else leaves_c1(std::false_type{}, std::false_type{});
           } else if (dr_ok) {   // acc <- round(RTO24(a * b + acc)), plain form
             auto dr_leaves = [&](auto f16) {
@@ -978,7 +1107,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
             if (g.dr_kind == 2) dr_leaves(std::true_type{});
             else dr_leaves(std::false_type{});
           } else if (rto_ok) {
-            if (g.sum_f16) leaves_c1(std::true_type{}, std::true_type{});
+            if (f16) leaves_c1(std::true_type{}, std::true_type{});
             else leaves_c1(std::false_type{}, std::true_type{});
           } else {
             float (&acc)[WMB][WNB][16] = nacc;
@@ -1159,8 +1288,54 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                             }
                 }
             }
+            if constexpr (CHUNK >= 8 && !CONV) {
             if (sum_ok) {   // nacc <- -round(p + acc) = round(nacc - p), the sum exact
-                if (g.sum_f16) {
+                if (kF16Leaves && f16) {   // one tied asm block per 16 outputs: no new registers
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj) exact_sub16x16(part[bi][bj], nacc[bi][bj]);
+                } else {
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r)
+                                nacc[bi][bj][r] = leaf_exact<false>(nacc[bi][bj][r] - part[bi][bj][r], g);
+                }
+            } else if (kF16Leaves && f16 && rto_ok) {
+                // binary16-shaped, every sum below 2^15, sums inexact in FP32: nacc <-
+                // binary16(nacc - p), one rounding of the exact difference
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj) mix_sub16x16(part[bi][bj], nacc[bi][bj]);
+            } else if (kDr && dr_ok) {   // nacc <- -round(RTO24(p + acc)), directed roundings
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj) leaves_dr16_neg(part[bi][bj], nacc[bi][bj], g);
+            } else if (!kDr && rto_ok) {   // nacc <- -round(RTO24(p + acc)), mantissa-only
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                        for (int r = 0; r < 16; ++r)
+                            nacc[bi][bj][r] = leaf_rto<false>(part[bi][bj][r], nacc[bi][bj][r], g);
+            } else {
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
+            }
+            } else {
+            // chunks of 2 and 4 (16 and 8 chunks per slice, unrolled), which measured faster
+            // with the compiler's own code for both binary16 leaves, and the convolution tiles
+            // (with the asm leaves the 64x64 chunk-8 one spills 146 registers)
+            if (sum_ok) {   // nacc <- -round(p + acc) = round(nacc - p), the sum exact
+                if (f16) {
 #pragma unroll
                     for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
@@ -1177,13 +1352,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                             for (int r = 0; r < 16; ++r)
                                 nacc[bi][bj][r] = leaf_exact<false>(nacc[bi][bj][r] - part[bi][bj][r], g);
                 }
-            } else if (kDr && dr_ok) {   // nacc <- -round(RTO24(p + acc)), directed roundings
-#pragma unroll
-                for (int bi = 0; bi < WMB; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < WNB; ++bj) leaves_dr16_neg(part[bi][bj], nacc[bi][bj], g);
-            } else if (!kDr && rto_ok) {   // nacc <- -round(RTO24(p + acc)) for any in-range sum
-                if (g.sum_f16) {
+            } else if (rto_ok) {   // nacc <- -round(RTO24(p + acc)) for any in-range sum
+                if (f16) {
 #pragma unroll
                     for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
@@ -1205,6 +1375,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
                     for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
+            }
             }
         }
         }

@@ -401,7 +401,8 @@ def test_directed_rounding_leaves(dev, kind, chunk):
     directed-rounding leaves (round-to-odd from the sum rounded down and up) for
     mantissa-only accumulators of every width and for binary16-shaped formats, at chunk 1 and
     on the chunk sums of chunks 8 and 32, against the oracle on every wave tile, and the same
-    tiles with those leaves off (0x8000).  Kinds: FP32 uniform(-1, 1) operands (the reference
+    tiles with those leaves off (0x8000) or with the mixed-precision FMA leaves of the
+    binary16-shaped formats off (0x20000).  Kinds: FP32 uniform(-1, 1) operands (the reference
     harness's lof_gemm inputs), MX-rounded e4m3 operands whose overflowing elements pass
     through unrounded, magnitudes across the FP32 range, sums that overflow to FLT_MAX | 1
     and inf, and products in the FP32 subnormal range."""
@@ -431,7 +432,7 @@ def test_directed_rounding_leaves(dev, kind, chunk):
         ref = _ref(A, B, oacc)
         what = f"{kind} chunk {chunk} m={acc.mant_bits if acc.fmt is None else 'fp16'}"
         for cfg in MFMA_CONFIGS:
-            for bits in (0, 0x8000):
+            for bits in (0, 0x8000, 0x20000):
                 assert_bits(_run(lof, A, B, acc, dev, config=cfg | bits), ref,
                             f"{what} config {cfg} bits {bits:#x}")
         assert_bits(_run(lof, A, B, acc, dev), ref, f"{what} default config")
@@ -482,3 +483,64 @@ def test_errors(dev):
         lof.gemm(a, torch.ones(5, 3, device=dev), lof.AccumSpec(mant_bits=10, mode=3))
     with pytest.raises(ValueError):
         lof.gemm(a, torch.ones(5, 3, device=dev), lof.AccumSpec(mant_bits=10, chunk=0))
+
+
+def _midpoint_traps(g, n, span):
+    """n exact sums x = T + d with T a binary16 midpoint in [2^-10, 2^10) and d far below
+    T's FP32 resolution, and a binary16 start value h a few ulps from T (x - h is then small
+    enough to be an FP32 value when span allows it)."""
+    e = g.integers(-10, 10, n)
+    q = np.exp2(e - 10.0)
+    m = g.integers(1024, 2048, n)
+    T = (m + 0.5) * q
+    h = (m + g.integers(-span, span + 1, n)) * q
+    d = g.choice([-1.0, 1.0], n) * np.exp2(e - 24.0 - g.integers(2, 4, n))
+    s = g.choice([-1.0, 1.0], n)
+    return s * T, s * h, s * d
+
+
+@pytest.mark.parametrize("fmt", [FP16_SAT, FP16_EXT])
+def test_mixed_fma_leaves_round_once(dev, fmt):
+    """Binary16-shaped accumulators: the mixed-precision FMA leaves must round the exact sum
+    once.  Each output n is built so that after its first chunk acc = h (a binary16 value)
+    and its second chunk brings x = acc + p to T + d, just off a binary16 midpoint T:
+    rounding the reference's round-to-odd sum goes to the side of d, while rounding the
+    sum to FP32 first would land on T and round to even.  Chunk 1: p = a b with a 24-bit a
+    (the product is inexact in FP32); chunk 8: p is an FP32 chunk sum.  Both against the
+    oracle, with the mixed FMA leaves on and off (0x20000), on every wave tile; the data
+    gives a double-rounding result that differs in a large share of outputs."""
+    import lofloat_amd as lof
+    from fractions import Fraction
+    g = np.random.default_rng(77)
+    N = 192
+    # chunk 1: K = 2, A = [1, a1], B[0] = h, B[1] = (x - h) / a1 rounded to FP32
+    T, h, d = _midpoint_traps(g, N, 3)
+    a1 = np.float32(1.0 + 1531 * 2.0 ** -23)
+    b1 = ((T + d - h) / np.float64(a1)).astype(np.float32)
+    A = np.array([[1.0, a1]], np.float32)
+    B = np.stack([h.astype(np.float32), b1])
+    # chunk 8: K = 16, acc = h after the first chunk, the second chunk's sum is p = x - h
+    T8, h8, d8 = _midpoint_traps(g, N, 3)
+    p8 = (T8 + d8 - h8).astype(np.float32)
+    assert np.all(p8.astype(np.float64) == T8 + d8 - h8)     # p is an FP32 value
+    A8 = np.zeros((1, 16), np.float32)
+    A8[0, 0] = 1.0
+    A8[0, 8] = 1.0
+    B8 = np.zeros((16, N), np.float32)
+    B8[0] = h8
+    B8[8] = p8
+    for chunk, (AA, BB) in ((1, (A, B)), (8, (A8, B8))):
+        acc, oacc = _accums(lof, fmt=fmt, chunk=chunk)
+        ref = _ref(AA, BB, oacc)
+        # the share of outputs where rounding through FP32 first would give other bits
+        first = BB[0].astype(np.float64) * float(AA[0, 0])
+        if chunk == 1:
+            xs = [Fraction(float(f)) + Fraction(float(a1)) * Fraction(float(b)) for f, b in zip(first, BB[1])]
+        else:
+            xs = [Fraction(float(f)) + Fraction(float(p)) for f, p in zip(first, BB[8])]
+        dbl = np.array([np.float16(np.float32(float(x))) for x in xs], np.float16).astype(np.float32)
+        assert np.mean(dbl != ref.ravel()) > 0.3
+        for cfg in MFMA_CONFIGS:
+            for bits in (0, 0x20000):
+                assert_bits(_run(lof, AA, BB, acc, dev, config=cfg | bits), ref,
+                            f"chunk {chunk} config {cfg} bits {bits:#x}")
