@@ -1256,16 +1256,10 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         for (int r = 0; r < 16; ++r)   // an exact zero sum is +0, as the chain gives
                             part[bi][bj][r] = part[bi][bj][r] + 0.0f;
             } else {
-#pragma unroll
-                for (int bi = 0; bi < WMB; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < WNB; ++bj)
-#pragma unroll
-                        for (int r = 0; r < 16; ++r) part[bi][bj][r] = 0.0f;
-#pragma unroll
-                for (int kq = 0; kq < CHUNK; ++kq) {
-                    const int kk = kb + kq;
-                    if (kk >= kmax) break;
+                // the fmaf chain from +0: its first k is fma(a, b, +0), which needs no zeroed
+                // registers (kb < kmax here, so that k exists)
+                auto chain_k = [&](int kk, auto first) {
+                    constexpr bool FIRST = decltype(first)::value;
                     float4 a4[WMB][4];
                     float b[WNB];
 #pragma unroll
@@ -1281,11 +1275,20 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                         for (int bj = 0; bj < WNB; ++bj)
 #pragma unroll
                             for (int rq = 0; rq < 4; ++rq) {
-                                part[bi][bj][rq * 4 + 0] = __builtin_fmaf(a4[bi][rq].x, b[bj], part[bi][bj][rq * 4 + 0]);
-                                part[bi][bj][rq * 4 + 1] = __builtin_fmaf(a4[bi][rq].y, b[bj], part[bi][bj][rq * 4 + 1]);
-                                part[bi][bj][rq * 4 + 2] = __builtin_fmaf(a4[bi][rq].z, b[bj], part[bi][bj][rq * 4 + 2]);
-                                part[bi][bj][rq * 4 + 3] = __builtin_fmaf(a4[bi][rq].w, b[bj], part[bi][bj][rq * 4 + 3]);
+                                const float av[4] = {a4[bi][rq].x, a4[bi][rq].y, a4[bi][rq].z,
+                                                     a4[bi][rq].w};
+#pragma unroll
+                                for (int x = 0; x < 4; ++x)
+                                    part[bi][bj][rq * 4 + x] = __builtin_fmaf(
+                                        av[x], b[bj], FIRST ? 0.0f : part[bi][bj][rq * 4 + x]);
                             }
+                };
+                chain_k(kb, std::true_type{});
+#pragma unroll
+                for (int kq = 1; kq < CHUNK; ++kq) {
+                    const int kk = kb + kq;
+                    if (kk >= kmax) break;
+                    chain_k(kk, std::false_type{});
                 }
             }
             if constexpr (CHUNK >= 8 && !CONV) {
