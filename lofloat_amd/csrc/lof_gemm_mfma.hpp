@@ -535,6 +535,10 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // directed-rounding leaves at chunk 1, and for the sums of chunks 8-32 in the DRC
     // instantiation; otherwise chunk sums keep the TwoSum round-to-odd leaves
     constexpr bool kDr = CHUNK == 1 || (DRC && CHUNK >= 8);
+    // checked binary16-held chunk-1 leaves of the 10-bit accumulator (below): the 64-row tiles
+    // only (the 128-row tiles hold twice the accumulators and are left as they are), and not
+    // the SUBN kernels, which no mantissa-only accumulator runs
+    constexpr bool kChk = CHUNK == 1 && WMB == 1 && !SUBN;
     // binary16-shaped accumulators run the SUBN kernels (their subnormal range lies above
     // FP32's), so the other kernels leave out their chunk-sum leaves -- except at chunk 32,
     // whose kernel measured 129 registers of spills without them (a different schedule)
@@ -868,6 +872,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         // and for a binary16-shaped one while every sum stays below 2^15; a lane whose
         // accumulators hold a NaN keeps the general leaf.
         bool sum_ok = false, rto_ok = false, h_ok = false, pk_ok = false, dr_ok = false;
+        bool h_chk = false;   // chunk 1: the checked held leaves (kChk)
         int hs = 0;
         const bool f16 = g.sum_f16;
         const bool try_fast = sfin && g.range_f != 0u && (exact || (CHUNK > 1 && g.sum_rto)) &&
@@ -940,11 +945,43 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     dr_ok = false;
                 }
             }
+            if constexpr (kChk) {
+                // 10-bit mantissa-only accumulator, products or sums inexact in FP32: held
+                // leaves on h = nacc 2^hs, checked.  Every scaled sum is below 2^15; where one
+                // is at least 2^-14 binary16 rounds it at 11 significant bits as the
+                // accumulator does, and a leaf result at most 2^-14 in magnitude (the only
+                // ones whose exact sum can lie below, underflows to zero included) sends the
+                // lanes of the wave on this path back over the slice with the directed-
+                // rounding leaves.  The start values must convert exactly (each nonzero one
+                // at least 2^-14 scaled) and be nonzero (zero start values would flag the
+                // slice's leading zero products).  The accumulator rounds FP32-subnormal
+                // values on a fixed grid and overflows past FP32's range, so every sum must
+                // stay below 2^127 (S <= 127) and the check must also catch every sum below
+                // 2^-126 (2^-14 scaled is at least 2^-126 when hs <= 112).
+                if (dr_ok && g.sum_h10 && amax < 0x7F800000u && !(g.sum_off & 32)) {
+                    const int h10 = 15 - S;
+                    const int emin = mnk != 0xFFFFFFFFu ? (int)((mnk + 1u) >> 23) - 127 : 1000;
+                    uint32_t amin = 0xFFFFFFFFu;
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                            for (int r = 0; r < 16; ++r)
+                                amin = min(amin, __float_as_uint(nacc[bi][bj][r]) & 0x7FFFFFFFu);
+                    if (amin != 0u && S <= g.sum_ehi && h10 <= 112 && bLsb + h10 >= -126 &&
+                        bTop + 1 + h10 <= 127 && emin + h10 >= -14) {
+                        h_chk = true;
+                        hs = h10;
+                        dr_ok = false;
+                    }
+                }
+            }
         }
 
         const int kmax = min(BK, g.K - k0);
         if constexpr (CHUNK == 1) {
-          set_form(sum_ok || pk_ok || h_ok || (rto_ok && !dr_ok));
+          set_form(sum_ok || pk_ok || h_ok || h_chk || (rto_ok && !dr_ok));
           // nacc <- -round(a * b + acc): exact sums (fma(-a, b, nacc) is -(a * b + acc)
           // exactly) or RTO leaves on the exact product a * b
           auto leaves_c1 = [&](auto f16, auto rto) {
@@ -1036,6 +1073,76 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
           if (pk_ok) {
             if (f16) pk_leaves(std::false_type{});
             else pk_leaves(std::true_type{});
+          } else if (kChk && h_chk) {   // the held leaves below, checked (see the lane test)
+            _Float16 h[WMB][WNB][16];
+            bool bad = false;
+#pragma unroll
+            for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        h[bi][bj][r] = (_Float16)__builtin_ldexpf(nacc[bi][bj][r], hs);
+            for (int kk = 0; kk < kmax; ++kk) {
+                float4 a4[WMB][4];
+                float b[WNB];
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int rq = 0; rq < 4; ++rq)
+                        a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
+#pragma unroll
+                for (int bj = 0; bj < WNB; ++bj)
+                    b[bj] = __builtin_ldexpf(Bs[kk * LDB_S + bn0 + bj * 32 + l32], hs);
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                        for (int rq = 0; rq < 4; ++rq) {
+                            _Float16* hv = &h[bi][bj][rq * 4];
+                            const float av[4] = {a4[bi][rq].x, a4[bi][rq].y, a4[bi][rq].z,
+                                                 a4[bi][rq].w};
+#pragma unroll
+                            for (int x = 0; x < 4; ++x) {
+                                hv[x] = mix_fnma16(av[x], b[bj], hv[x]);
+                                bad |= (hv[x] <= (_Float16)6.103515625e-05f) &
+                                       (hv[x] >= (_Float16)-6.103515625e-05f);
+                            }
+                        }
+            }
+            if (__builtin_amdgcn_ballot_w64(bad) == 0) {
+#pragma unroll
+                for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj)
+#pragma unroll
+                        for (int r = 0; r < 16; ++r)
+                            nacc[bi][bj][r] = __builtin_ldexpf((float)h[bi][bj][r], -hs);
+            } else {   // a result within binary16's subnormal range: the directed-rounding
+                       // leaves over the slice from the unchanged start values, plain form
+                set_form(false);
+                for (int kk = 0; kk < kmax; ++kk) {
+                    float4 a4[WMB][4];
+                    float b[WNB];
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi)
+#pragma unroll
+                        for (int rq = 0; rq < 4; ++rq)
+                            a4[bi][rq] = *(const float4*)&As[kk * LDA_S + am0 + bi * 32 + rq * 8 + lhi * 4];
+#pragma unroll
+                    for (int bj = 0; bj < WNB; ++bj) b[bj] = Bs[kk * LDB_S + bn0 + bj * 32 + l32];
+#pragma unroll
+                    for (int bi = 0; bi < WMB; ++bi) {
+                        const float av[16] = {a4[bi][0].x, a4[bi][0].y, a4[bi][0].z, a4[bi][0].w,
+                                              a4[bi][1].x, a4[bi][1].y, a4[bi][1].z, a4[bi][1].w,
+                                              a4[bi][2].x, a4[bi][2].y, a4[bi][2].z, a4[bi][2].w,
+                                              a4[bi][3].x, a4[bi][3].y, a4[bi][3].z, a4[bi][3].w};
+#pragma unroll
+                        for (int bj = 0; bj < WNB; ++bj) leaves_dr16<false>(av, b[bj], nacc[bi][bj], g);
+                    }
+                }
+            }
           } else if (h_ok) {   // h <- binary16(h - a b 2^hs), h = nacc 2^hs (exact both ways)
             _Float16 h[WMB][WNB][16];
 #pragma unroll

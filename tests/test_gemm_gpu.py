@@ -395,7 +395,7 @@ def test_rto_leaves(dev, chunk):
 
 
 @pytest.mark.parametrize("chunk", [1, 8, 32])
-@pytest.mark.parametrize("kind", ["uniform", "mx", "wide", "overflow", "tiny"])
+@pytest.mark.parametrize("kind", ["uniform", "mx", "wide", "overflow", "tiny", "split"])
 def test_directed_rounding_leaves(dev, kind, chunk):
     """Products or chunk sums that FP32 cannot hold exactly (24-bit significands): the
     directed-rounding leaves (round-to-odd from the sum rounded down and up) for
@@ -405,9 +405,11 @@ def test_directed_rounding_leaves(dev, kind, chunk):
     binary16-shaped formats off (0x20000).  Kinds: FP32 uniform(-1, 1) operands (the reference
     harness's lof_gemm inputs), MX-rounded e4m3 operands whose overflowing elements pass
     through unrounded, magnitudes across the FP32 range, sums that overflow to FLT_MAX | 1
-    and inf, and products in the FP32 subnormal range."""
+    and inf, products in the FP32 subnormal range, and columns 2^40 apart (the checked
+    binary16-held leaves of the 10-bit accumulator fall back to the directed ones)."""
     import lofloat_amd as lof
-    g = np.random.default_rng({"uniform": 1, "mx": 2, "wide": 3, "overflow": 4, "tiny": 5}[kind])
+    g = np.random.default_rng({"uniform": 1, "mx": 2, "wide": 3, "overflow": 4, "tiny": 5,
+                               "split": 6}[kind])
     M, N, K = 80, 96, 192
     if kind == "uniform":
         A = g.uniform(-1, 1, (M, K)).astype(np.float32)
@@ -423,9 +425,14 @@ def test_directed_rounding_leaves(dev, kind, chunk):
     elif kind == "overflow":
         A = (g.standard_normal((M, K)) * 2.0 ** 62).astype(np.float32)
         B = (g.standard_normal((K, N)) * 2.0 ** 64).astype(np.float32)
-    else:
+    elif kind == "tiny":
         A = (g.standard_normal((M, K)) * 2.0 ** -70).astype(np.float32)
         B = (g.standard_normal((K, N)) * 2.0 ** -62).astype(np.float32)
+    else:   # columns 2^40 apart within each lane's pair: the small ones scale into binary16's
+        # subnormal range, which sends the checked 10-bit leaves back to the directed ones
+        A = g.uniform(-1, 1, (M, K)).astype(np.float32)
+        B = g.uniform(-1, 1, (K, N)).astype(np.float32)
+        B[:, (np.arange(N) // 32) % 2 == 1] *= np.float32(2.0 ** -40)
     accs = [_accums(lof, m, chunk) for m in (0, 5, 10, 19, 20, 21, 22, 23)]
     accs += [_accums(lof, fmt=FP16_SAT, chunk=chunk), _accums(lof, fmt=FP16_EXT, chunk=chunk)]
     for acc, oacc in accs:
