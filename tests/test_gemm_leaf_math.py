@@ -475,3 +475,45 @@ def test_directed_rounding_leaves():
             assert _bits(F32(-_dr_round(tn, m, None))) == _bits(ref_n), (p, acc, m)
         checked += 1
     assert checked > 2800 and odd > 1000, (checked, odd)
+
+
+def test_checked_binary16_held_leaves_10bit():
+    """The checked binary16-held chunk-1 leaves of the 10-bit mantissa-only accumulator: with
+    h = -acc 2^hs (a binary16 value) and any FP32 a, b, the leaf computes binary16's single
+    rounding of the exact h - a b 2^hs (the mixed-precision FMA).  Wherever that result is
+    above 2^-14 in magnitude, scaling it back must give the reference leaf
+    round(RTO24(a b + acc)) negated; results at most 2^-14 (which the kernel flags, sending
+    the slice to the directed-rounding leaves) are where the two may differ, and the data
+    must reach that case too."""
+    from fractions import Fraction
+    g = np.random.default_rng(1010)
+    checked = flagged = differ_flagged = 0
+    for it in range(4000):
+        hs = int(g.integers(-20, 40))
+        # a 10-bit accumulator value whose scaled image is a binary16-normal value
+        acc = F32(oracle.virtual_round_mant_scalar(
+            float(g.standard_normal() * 2.0 ** (g.integers(-14, 14) - hs)), 10))
+        if acc == 0 or not (2.0 ** -14 <= abs(float(acc)) * 2.0 ** hs < 2.0 ** 15):
+            continue
+        a = F32(g.uniform(-1, 1) * 2.0 ** g.integers(-8, 8))
+        if it % 3 == 0:   # near-cancellation: a b close to -acc
+            b = F32(-float(acc) / float(a) * (1 + g.standard_normal() * 2.0 ** -g.integers(8, 30)))
+        else:
+            b = F32(g.uniform(-1, 1) * 2.0 ** (g.integers(-8, 8)))
+        x = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(acc))
+        if abs(x) * Fraction(2) ** hs >= 2 ** 15 or abs(x) >= Fraction(2) ** 127:
+            continue
+        h_old = -Fraction(float(acc)) * Fraction(2) ** hs
+        hx = h_old - Fraction(float(a)) * Fraction(float(b)) * Fraction(2) ** hs
+        h_new = _round_binary16(hx) if hx != 0 else 0.0
+        rto = oracle.rto_fma(float(a), float(b), float(acc))
+        ref = F32(oracle.virtual_round_mant_scalar(rto, 10))
+        got = F32(-h_new * 2.0 ** -hs) if h_new != 0 else None
+        if abs(h_new) <= 2.0 ** -14:
+            flagged += 1
+            if got is None or _bits(got) != _bits(ref):
+                differ_flagged += 1
+            continue
+        assert _bits(got) == _bits(ref), (a, b, acc, hs, got, ref)
+        checked += 1
+    assert checked > 1500 and flagged > 50 and differ_flagged > 0, (checked, flagged, differ_flagged)
