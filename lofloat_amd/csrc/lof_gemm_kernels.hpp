@@ -67,6 +67,26 @@ struct ConvGeom {
     uint32_t PQ, D1;
 };
 
+// Global-memory access for the GEMM tiles.  Their operands arrive as generic pointers
+// (members of GemmArgs), which the compiler would read and write with FLAT instructions;
+// those count against the LDS wait counter as well, so every wait for an LDS read would also
+// wait for the next slice's prefetch.  Through the global address space they are
+// global_load / global_store.
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+// float4 (a class type, whose copy would go through a generic reference) as a native vector
+__device__ __forceinline__ float4 gld4(const float* p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = *(const __attribute__((address_space(1))) v4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct GemmArgs {
     int32_t M, N, K;
     const float* A;
@@ -161,12 +181,29 @@ __device__ __noinline__ uint32_t rto24_bits(double p, double acc, double s, doub
     return rz | 1u;
 }
 
-__device__ __noinline__ double leaf_general(double p, double acc, double s, double e,
-                                            const GemmArgs& g) {
+// What the out-of-line general leaf reads, for passing it by value.  The general leaf and
+// leaf_slow take either this or a reference to the kernel's GemmArgs (P).  A reference gives
+// GemmArgs a private-memory copy whose fields the kernel reads back after each call instead
+// of holding them in registers across it: fewer registers (the VALU tiles, the 128-row MFMA
+// tiles and the short-chunk tiles keep their occupancy that way), but scratch reads in the
+// tile loop, whose waits (the vector-memory counter) also wait for the next slice's
+// in-flight prefetch.
+struct LeafGen {
+    int32_t acc_kind, acc_mode, acc_r;
+    uint32_t range;   // nonzero while the fast path is on (leaf_zero's condition)
+    FmtDev accF;
+};
+
+template <class P>   // const GemmArgs& or LeafGen
+__device__ __noinline__ double leaf_general(double p, double acc, double s, double e, P g) {
     const uint32_t F = rto24_bits(p, acc, s, e);
     const uint32_t R = g.acc_kind == 0 ? vr_mant_rt(F, g.acc_mode, g.acc_r, 0, 0u)
                                        : vr_fmt_rt(F, g.accF, 0u);
     return (double)__uint_as_float(R);
+}
+
+__device__ __forceinline__ LeafGen leaf_gen(const GemmArgs& g) {
+    return LeafGen{g.acc_kind, g.acc_mode, g.acc_r, g.range, g.accF};
 }
 
 // One emulated accumulation step with exact product/partial sum p, fast path only: returns
@@ -211,11 +248,12 @@ __device__ __forceinline__ bool leaf_zero(double p, double acc, const GemmArgs& 
     return true;
 }
 
-__device__ __noinline__ double leaf_slow(double p, double acc, const GemmArgs& g) {
+template <class P>   // const GemmArgs& or LeafGen
+__device__ __noinline__ double leaf_slow(double p, double acc, P g) {
     const double s = p + acc;
     const double bb = s - p;
     const double e = (p - (s - bb)) + (acc - bb);
-    return leaf_general(p, acc, s, e, g);
+    return leaf_general<P>(p, acc, s, e, g);
 }
 
 // One emulated accumulation step (fast path, else the general path).  The fast path's
@@ -224,7 +262,7 @@ __device__ __noinline__ double leaf_slow(double p, double acc, const GemmArgs& g
 __device__ __forceinline__ double leaf(double p, double acc, const GemmArgs& g) {
     double r;
     const bool ok = g.subn ? leaf_fast<true>(p, acc, g, r) : leaf_fast<false>(p, acc, g, r);
-    if (__builtin_expect(!ok, 0) && !leaf_zero(p, acc, g, r)) r = leaf_slow(p, acc, g);
+    if (__builtin_expect(!ok, 0) && !leaf_zero(p, acc, g, r)) r = leaf_slow<const GemmArgs&>(p, acc, g);
     return r;
 }
 
@@ -250,7 +288,7 @@ __device__ __forceinline__ void leaves_d(const double* p, double* acc, const Gem
             for (int o = g0; o < g0 + G && o < N; ++o) {
                 double z;
                 if ((slow >> (o - g0)) & 1u)
-                    acc[o] = leaf_zero(p[o], acc[o], g, z) ? z : leaf_slow(p[o], acc[o], g);
+                    acc[o] = leaf_zero(p[o], acc[o], g, z) ? z : leaf_slow<const GemmArgs&>(p[o], acc[o], g);
             }
         }
     }
@@ -289,7 +327,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
             const int gm = m0 + m, gk = k0 + k;
             float v = 0.0f;
             if (gm < g.M && gk < g.K)
-                v = g.ta ? g.A[(int64_t)gk * g.lda + gm] : g.A[(int64_t)gm * g.lda + gk];
+                v = gld(g.ta ? g.A + (int64_t)gk * g.lda + gm : g.A + (int64_t)gm * g.lda + gk);
             As[k * LDA_S + m] = v;
         }
 #pragma unroll
@@ -301,7 +339,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
             const int gn = n0 + n, gk = k0 + k;
             float v = 0.0f;
             if (gn < g.N && gk < g.K)
-                v = g.tb ? g.B[(int64_t)gn * g.ldb + gk] : g.B[(int64_t)gk * g.ldb + gn];
+                v = gld(g.tb ? g.B + (int64_t)gn * g.ldb + gk : g.B + (int64_t)gk * g.ldb + gn);
             Bs[k * LDB_S + n] = v;
         }
         __syncthreads();
@@ -364,9 +402,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs& g, int tile_m, int til
             if (gn >= g.N) continue;
             const float a = (float)acc[i][j];
             float d;
-            if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * g.C[(int64_t)gm * g.ldc + gn]);
+            if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * gld(g.C + (int64_t)gm * g.ldc + gn));
             else d = g.alpha * a;
-            g.D[(int64_t)gm * g.ldd + gn] = d;
+            gst(g.D + (int64_t)gm * g.ldd + gn, d);
         }
     }
 }

@@ -74,6 +74,15 @@ __device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
     return v;
 }
 
+// The float leaf's slow path: the exact-zero leaf, else the general leaf (the double fast
+// leaf, which gives the same value where it applies, is left out: it reads more of GemmArgs).
+// The second form is the same through a reference to the kernel's arguments (see leaf_slow).
+__device__ __noinline__ float leaf_f_slow(float p, float acc, LeafGen L) {
+    const double dp = p, da = acc;
+    if (L.range != 0u && dp + da == 0.0)   // leaf_zero: RTO24's sign of an exact zero
+        return (__double_as_longlong(dp) | __double_as_longlong(da)) == 0 ? 0.0f : -0.0f;
+    return (float)leaf_slow<LeafGen>(dp, da, L);
+}
 __device__ __noinline__ float leaf_f_slow(float p, float acc, const GemmArgs& g) {
     return (float)leaf((double)p, (double)acc, g);
 }
@@ -438,8 +447,9 @@ __device__ __forceinline__ void leaves_dr16_neg(const f32x16& p, float (&nacc)[1
 // Leaves of one 32x32 block's 16 outputs per lane, branch-free in the common case: fast
 // results are committed where valid, outputs needing the general path are recorded in a bit
 // mask, and one wave-uniform branch runs the general path for them (acc still holds their
-// old value).
-template <bool SUBN>
+// old value).  BYVAL passes the general path what it needs by value (LeafGen), so the
+// kernel's GemmArgs is never referenced from out-of-line code (see leaf_slow).
+template <bool SUBN, bool BYVAL = false>
 __device__ __forceinline__ void leaves_f16(const f32x16& p, float* acc, const GemmArgs& g) {
     uint32_t slow = 0;
 #pragma unroll
@@ -454,7 +464,9 @@ __device__ __forceinline__ void leaves_f16(const f32x16& p, float* acc, const Ge
         for (int o = 0; o < 16; ++o) {
             float z;
             if ((slow >> o) & 1u)
-                acc[o] = leaf_f_zero(p[o], acc[o], g, z) ? z : leaf_f_slow(p[o], acc[o], g);
+                if (leaf_f_zero(p[o], acc[o], g, z)) acc[o] = z;
+                else if constexpr (BYVAL) acc[o] = leaf_f_slow(p[o], acc[o], leaf_gen(g));
+                else acc[o] = leaf_f_slow(p[o], acc[o], g);
         }
     }
 }
@@ -518,7 +530,10 @@ __device__ __forceinline__ bool slice_bounds(uint32_t amax, uint32_t mnk, uint32
 // round-to-odd leaves: the fast leaf for mantissa-only accumulators of 20 or more bits, which
 // no other leaf covers.  An instantiation of its own, since both kinds in one kernel spill
 // and slow the exact-sum workloads by a fifth.
-template <int WMB, int WNB, int CHUNK, bool SUBN, bool CONV = false, bool DRC = false>
+// GMEM: g lives in global memory (a task of the grouped kernel), not in the kernel's
+// arguments, so a reference to it costs no private copy.
+template <int WMB, int WNB, int CHUNK, bool SUBN, bool CONV = false, bool DRC = false,
+          bool GMEM = false>
 __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, int tile_n,
                                                int tid, float* smem) {
     using T = MfmaTile<WMB, WNB>;
@@ -543,6 +558,13 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // FP32's), so the other kernels leave out their chunk-sum leaves -- except at chunk 32,
     // whose kernel measured 129 registers of spills without them (a different schedule)
     constexpr bool kF16Leaves = (SUBN || CHUNK >= 32) && !DRC;
+    // the general leaf's arguments by value in the 64-row tiles of chunks 8-32: their tile
+    // loop then holds GemmArgs in registers instead of reading it back from scratch (whose
+    // waits also wait for the in-flight prefetch; measured 30% faster at chunk 32).  The
+    // other tiles keep the reference, which costs them fewer registers (measured: the DRC
+    // tile of chunk 8 loses 5% by the value, the chunk-32 one gains 12%; chunk 16 keeps the
+    // reference with chunk 8).
+    constexpr bool kLeafByVal = WMB == 1 && CHUNK >= 8 && !GMEM && (!DRC || CHUNK == 32);
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -635,7 +657,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 const int ih = cv_ih0[j] + rr, iw = cv_iw0[j] + ss;
                 const bool in = gk < g.K && (unsigned)ih < (unsigned)cv.H &&
                                 (unsigned)iw < (unsigned)cv.W;
-                const float v = g.B[in ? cv_base[j] + koff : 0];   // offset 0 when outside
+                const float v = gld(g.B + (in ? cv_base[j] + koff : 0));   // offset 0 when outside
                 e[j] = in ? v : 0.0f;
             }
             vb[it] = make_float4(e[0], e[1], e[2], e[3]);
@@ -650,7 +672,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 a_coord(it, 0, m, k);
                 const float* p = g.ta ? g.A + (int64_t)(k0 + k) * g.lda + (m0 + m)
                                       : g.A + (int64_t)(m0 + m) * g.lda + (k0 + k);
-                va[it] = *(const float4*)p;
+                va[it] = gld4(p);
             }
         } else {   // edge: element loads from clamped addresses, zero outside the matrix
 #pragma unroll
@@ -662,7 +684,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     a_coord(it, j, m, k);
                     const int gm = m0 + m, gk = k0 + k;
                     const int cm = min(gm, g.M - 1), ck = min(gk, g.K - 1);
-                    const float v = g.ta ? g.A[(int64_t)ck * g.lda + cm] : g.A[(int64_t)cm * g.lda + ck];
+                    const float v = gld(g.ta ? g.A + (int64_t)ck * g.lda + cm : g.A + (int64_t)cm * g.lda + ck);
                     e[j] = (gm < g.M && gk < g.K) ? v : 0.0f;
                 }
                 va[it] = make_float4(e[0], e[1], e[2], e[3]);
@@ -677,7 +699,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 b_coord(it, 0, n, k);
                 const float* p = g.tb ? g.B + (int64_t)(n0 + n) * g.ldb + (k0 + k)
                                       : g.B + (int64_t)(k0 + k) * g.ldb + (n0 + n);
-                vb[it] = *(const float4*)p;
+                vb[it] = gld4(p);
             }
         } else {
 #pragma unroll
@@ -689,7 +711,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                     b_coord(it, j, n, k);
                     const int gn = n0 + n, gk = k0 + k;
                     const int cn = min(gn, g.N - 1), ck = min(gk, g.K - 1);
-                    const float v = g.tb ? g.B[(int64_t)cn * g.ldb + ck] : g.B[(int64_t)ck * g.ldb + cn];
+                    const float v = gld(g.tb ? g.B + (int64_t)cn * g.ldb + ck : g.B + (int64_t)ck * g.ldb + cn);
                     e[j] = (gn < g.N && gk < g.K) ? v : 0.0f;
                 }
                 vb[it] = make_float4(e[0], e[1], e[2], e[3]);
@@ -1243,7 +1265,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                                 pv[rq * 4 + 2] = __fmul_rn(a4[bi][rq].z, b[bj]);
                                 pv[rq * 4 + 3] = __fmul_rn(a4[bi][rq].w, b[bj]);
                             }
-                            leaves_f16<SUBN>(pv, acc[bi][bj], g);
+                            leaves_f16<SUBN, kLeafByVal>(pv, acc[bi][bj], g);
                         }
                 } else {   // products wider than FP32: exact in double, one leaf at a time
 #pragma unroll
@@ -1438,7 +1460,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
 #pragma unroll
                 for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
+                    for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN, kLeafByVal>(part[bi][bj], acc[bi][bj], g);
             }
             } else {
             // chunks of 2 and 4 (16 and 8 chunks per slice, unrolled), which measured faster
@@ -1484,7 +1506,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
 #pragma unroll
                 for (int bi = 0; bi < WMB; ++bi)
 #pragma unroll
-                    for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN>(part[bi][bj], acc[bi][bj], g);
+                    for (int bj = 0; bj < WNB; ++bj) leaves_f16<SUBN, kLeafByVal>(part[bi][bj], acc[bi][bj], g);
             }
             }
         }
@@ -1511,9 +1533,9 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 if (gm >= g.M || gn >= g.N) continue;
                 const float a = negd ? -nacc[bi][bj][r] : nacc[bi][bj][r];
                 float d;
-                if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * g.C[(int64_t)gm * row_stride_c + col]);
+                if (g.beta != 0.0f) d = __builtin_fmaf(g.alpha, a, g.beta * gld(g.C + (int64_t)gm * row_stride_c + col));
                 else d = g.alpha * a;
-                g.D[(int64_t)gm * row_stride_d + col] = d;
+                gst(g.D + (int64_t)gm * row_stride_d + col, d);
             }
         }
 }

@@ -116,7 +116,7 @@ __device__ __noinline__ void grouped_mx(const MxArgs& a, int mode, int rsrc, int
 template <int WMB, int WNB, int CHUNK, bool SUBN>
 __device__ __forceinline__ void g_mfma_tile(const GemmArgs& g, int tm, int tn, int tid,
                                          float* smem) {
-    gemm_tile_mfma<WMB, WNB, CHUNK, SUBN>(g, tm, tn, tid, smem);
+    gemm_tile_mfma<WMB, WNB, CHUNK, SUBN, false, false, true>(g, tm, tn, tid, smem);
 }
 
 template <int WMB, int WNB>
