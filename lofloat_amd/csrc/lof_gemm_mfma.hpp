@@ -602,15 +602,32 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     //   ta = 1 (A stored K x M, m contiguous): k = f / (BM / 4), m = 4 * (f % (BM / 4)) .. + 3
     // and likewise for B (tb = 0: n contiguous; tb = 1: k contiguous).
     float4 va[NA4], vb[NB4];
-    auto a_coord = [&](int it, int j, int& m, int& k) {
-        const int f = tid + it * 256;
-        if (g.ta) { k = f / (BM / 4); m = 4 * (f % (BM / 4)) + j; }
-        else { m = f / 8; k = 4 * (f % 8) + j; }
+    // Float4 `it` lies 256 float4s past float4 0, which moves it by a whole number of rows
+    // (256 is a multiple of 8, BM / 4 and BN / 4): per thread, the coordinates of float4 0,
+    // and per float4 a wave-uniform step.  (Assigned through references in two branches, the
+    // coordinate pair was kept in private memory, and its scratch reads -- whose waits also
+    // wait for the in-flight prefetch -- sat in every slice's store.)
+    const int a_mb = g.ta ? 4 * (tid % (BM / 4)) : tid / 8;
+    const int a_kb = g.ta ? tid / (BM / 4) : 4 * (tid % 8);
+    const int b_nb = g.tb ? tid / 8 : 4 * (tid % (BN / 4));
+    const int b_kb = g.tb ? 4 * (tid % 8) : tid / (BN / 4);
+    auto a_coord = [&](int it, int j) -> int2 {   // (m, k)
+        return make_int2(a_mb + (g.ta ? j : 32 * it), a_kb + (g.ta ? it * (1024 / BM) : j));
     };
-    auto b_coord = [&](int it, int j, int& n, int& k) {
-        const int f = tid + it * 256;
-        if (g.tb) { n = f / 8; k = 4 * (f % 8) + j; }
-        else { k = f / (BN / 4); n = 4 * (f % (BN / 4)) + j; }
+    auto b_coord = [&](int it, int j) -> int2 {   // (n, k)
+        return make_int2(b_nb + (g.tb ? 32 * it : j), b_kb + (g.tb ? j : it * (1024 / BN)));
+    };
+    // The edge loads' element coordinates, from a copy of tid the compiler cannot see
+    // through: otherwise it hoists their loop-invariant parts out of the tile loop for
+    // every element, and the registers they hold spill into the interior slices' path.
+    auto edge_coord = [&](bool isa, int it, int j) -> int2 {
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        const int f = t + it * 256;
+        if (isa) return g.ta ? make_int2(4 * (f % (BM / 4)) + j, f / (BM / 4))
+                             : make_int2(f / 8, 4 * (f % 8) + j);
+        return g.tb ? make_int2(f / 8, 4 * (f % 8) + j)
+                    : make_int2(4 * (f % (BN / 4)) + j, f / (BN / 4));
     };
     // Convolution: this thread's four B columns n = n0 + 4 * (tid % (BN / 4)) + j are the
     // same in every slice (256 is a multiple of BN / 4); their output image, first input row
@@ -639,8 +656,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         const ConvGeom& cv = g.cv;
 #pragma unroll
         for (int it = 0; it < NB4; ++it) {
-            int n, k;
-            b_coord(it, 0, n, k);
+            const int2 nk = b_coord(it, 0);
+            const int n = nk.x, k = nk.y;
             const int gk = k0 + k;
             const uint32_t d1 = fast_div((uint32_t)gk, cv.div_d1);
             const uint32_t rem = (uint32_t)gk - d1 * cv.D1;
@@ -668,8 +685,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         if (whole) {   // whole slice inside: float4 loads
 #pragma unroll
             for (int it = 0; it < NA4; ++it) {
-                int m, k;
-                a_coord(it, 0, m, k);
+                const int2 mk = a_coord(it, 0);
+                const int m = mk.x, k = mk.y;
                 const float* p = g.ta ? g.A + (int64_t)(k0 + k) * g.lda + (m0 + m)
                                       : g.A + (int64_t)(m0 + m) * g.lda + (k0 + k);
                 va[it] = gld4(p);
@@ -680,8 +697,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 float e[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    int m, k;
-                    a_coord(it, j, m, k);
+                    const int2 mk = edge_coord(true, it, j);
+                    const int m = mk.x, k = mk.y;
                     const int gm = m0 + m, gk = k0 + k;
                     const int cm = min(gm, g.M - 1), ck = min(gk, g.K - 1);
                     const float v = gld(g.ta ? g.A + (int64_t)ck * g.lda + cm : g.A + (int64_t)cm * g.lda + ck);
@@ -695,8 +712,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         } else if (whole) {
 #pragma unroll
             for (int it = 0; it < NB4; ++it) {
-                int n, k;
-                b_coord(it, 0, n, k);
+                const int2 nk = b_coord(it, 0);
+                const int n = nk.x, k = nk.y;
                 const float* p = g.tb ? g.B + (int64_t)(n0 + n) * g.ldb + (k0 + k)
                                       : g.B + (int64_t)(k0 + k) * g.ldb + (n0 + n);
                 vb[it] = gld4(p);
@@ -707,8 +724,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                 float e[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    int n, k;
-                    b_coord(it, j, n, k);
+                    const int2 nk = edge_coord(false, it, j);
+                    const int n = nk.x, k = nk.y;
                     const int gn = n0 + n, gk = k0 + k;
                     const int cn = min(gn, g.N - 1), ck = min(gk, g.K - 1);
                     const float v = gld(g.tb ? g.B + (int64_t)cn * g.ldb + ck : g.B + (int64_t)ck * g.ldb + cn);
@@ -726,8 +743,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         SliceStat<kKey> sa, sb;
 #pragma unroll
         for (int it = 0; it < NA4; ++it) {
-            int m, k;
-            a_coord(it, 0, m, k);
+            const int2 mk = a_coord(it, 0);
+            const int m = mk.x, k = mk.y;
             if (g.ta) {
                 *(float4*)&As[k * LDA_S + m] = va[it];
             } else {
@@ -740,8 +757,8 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         }
 #pragma unroll
         for (int it = 0; it < NB4; ++it) {
-            int n, k;
-            b_coord(it, 0, n, k);
+            const int2 nk = b_coord(it, 0);
+            const int n = nk.x, k = nk.y;
             if (g.tb) {
                 Bs[(k + 0) * LDB_S + n] = vb[it].x;
                 Bs[(k + 1) * LDB_S + n] = vb[it].y;
