@@ -258,11 +258,15 @@ __device__ __noinline__ double leaf_slow(double p, double acc, P g) {
 
 // One emulated accumulation step (fast path, else the general path).  The fast path's
 // domain reaches into the format's subnormal range when g.subn is set, so the rounding
-// position must follow it there.
+// position must follow it there.  BYVAL: the general path's arguments by value (LeafGen).
+template <bool BYVAL = false>
 __device__ __forceinline__ double leaf(double p, double acc, const GemmArgs& g) {
     double r;
     const bool ok = g.subn ? leaf_fast<true>(p, acc, g, r) : leaf_fast<false>(p, acc, g, r);
-    if (__builtin_expect(!ok, 0) && !leaf_zero(p, acc, g, r)) r = leaf_slow<const GemmArgs&>(p, acc, g);
+    if (__builtin_expect(!ok, 0) && !leaf_zero(p, acc, g, r)) {
+        if constexpr (BYVAL) r = leaf_slow<LeafGen>(p, acc, leaf_gen(g));
+        else r = leaf_slow<const GemmArgs&>(p, acc, g);
+    }
     return r;
 }
 

@@ -558,13 +558,15 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // FP32's), so the other kernels leave out their chunk-sum leaves -- except at chunk 32,
     // whose kernel measured 129 registers of spills without them (a different schedule)
     constexpr bool kF16Leaves = (SUBN || CHUNK >= 32) && !DRC;
-    // the general leaf's arguments by value in the 64-row tiles of chunks 8-32: their tile
-    // loop then holds GemmArgs in registers instead of reading it back from scratch (whose
-    // waits also wait for the in-flight prefetch; measured 30% faster at chunk 32).  The
-    // other tiles keep the reference, which costs them fewer registers (measured: the DRC
-    // tile of chunk 8 loses 5% by the value, the chunk-32 one gains 12%; chunk 16 keeps the
-    // reference with chunk 8).
-    constexpr bool kLeafByVal = WMB == 1 && CHUNK >= 8 && !GMEM && (!DRC || CHUNK == 32);
+    // the general leaf's arguments by value in the 64-row tiles: their tile loop then holds
+    // GemmArgs in registers instead of reading it back from scratch (whose waits also wait
+    // for the in-flight prefetch; measured 30% faster at chunk 32, 10-25% at chunk 1).  The
+    // other tiles keep the reference, which costs them fewer registers (measured: the
+    // 128-row tiles lose occupancy by the value, the 64x128 tile at chunks 2 and 4 loses up
+    // to 20%, the DRC tile of chunk 8 5% (chunk 16 keeps the reference with it), while the
+    // chunk-32 DRC tile gains 12%).
+    constexpr bool kLeafByVal = WMB == 1 && !GMEM && (!DRC || CHUNK == 32) &&
+                                (WNB == 1 || CHUNK == 1 || CHUNK >= 8);
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -1296,7 +1298,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
                                                   a4[bi][3].x, a4[bi][3].y, a4[bi][3].z, a4[bi][3].w};
 #pragma unroll
                             for (int o = 0; o < 16; ++o)
-                                acc[bi][bj][o] = (float)leaf((double)av[o] * bd,
+                                acc[bi][bj][o] = (float)leaf<kLeafByVal>((double)av[o] * bd,
                                                              (double)acc[bi][bj][o], g);
                         }
                 }
