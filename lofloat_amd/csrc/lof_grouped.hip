@@ -2,9 +2,11 @@
 //
 // The plan uploads one GroupTask per job and the first tile index of each job (jobs are
 // ordered GEMMs first, largest first, then rounding jobs, largest first).  GEMMs with
-// k-chunk 8, 16 or 32 use the config's MFMA chunk-sum tile (64 x 64 or 128 x 64), the
-// others the config's VALU tile.  The kernel is held to two workgroups per CU (launch
-// bounds), which the 128 x 64 MFMA tile would otherwise drop to one.  A fixed grid of
+// k-chunk 8, 16 or 32 use the config's MFMA chunk-sum tile (64 x 64, 128 x 64 or 64 x 128),
+// the others the config's VALU tile.  The kernel is held to two workgroups per CU (launch
+// bounds), which the 128 x 64 MFMA tile would otherwise drop to one, and the configs with
+// the 64 x 128 tile to three, as its single-GEMM kernel (measured 11.9 against 12.9 ms on
+// the 64-job benchmark mix, although the bound costs spills).  A fixed grid of
 // workgroups (as many as fit on the device at once) repeatedly claims the next tile with
 // an atomic counter, finds its job by binary search over the start indices, and runs the
 // job's tile routine: the same device routines as the single-job kernels, dispatched once
@@ -133,7 +135,7 @@ __device__ __forceinline__ void grouped_mfma(const GemmArgs& g, int tm, int tn, 
 }
 
 template <class C>
-__global__ __launch_bounds__(256, 2) void k_grouped(const GroupTask* __restrict__ tasks,
+__global__ __launch_bounds__(256, C::mfma_wnb == 2 ? 3 : 2) void k_grouped(const GroupTask* __restrict__ tasks,
                                                  const int64_t* __restrict__ starts, int ntasks,
                                                  int64_t total, unsigned long long* counters) {
     constexpr int kValuSmem = gemm_smem_floats<C::bm, C::bn, C::bk>();
