@@ -289,7 +289,11 @@ typedef struct lof_group_item {
 typedef struct lof_grouped lof_grouped;
 
 /* Validate the items and upload the work list.  config: -1 = default (or the tuned
- * choice from lof_grouped_autotune), else < lof_grouped_num_configs(). */
+ * choice from lof_grouped_autotune), else < lof_grouped_num_configs().  A GEMM item with
+ * at least as many tiles as the persistent grid has workgroups fills the device by itself:
+ * lof_grouped_run launches it through the single-GEMM kernel (whose tile is faster than
+ * the combined kernel's) ahead of the persistent launch that runs every other item.  The
+ * outputs are bit-identical either way. */
 LOF_API lof_status lof_grouped_create(const lof_group_item* items, int32_t count, int32_t config,
                                       lof_grouped** plan);
 LOF_API lof_status lof_grouped_run(lof_grouped* plan, lof_stream_t stream);
@@ -299,7 +303,8 @@ LOF_API lof_status lof_grouped_run_separate(lof_grouped* plan, lof_stream_t stre
 LOF_API lof_status lof_grouped_destroy(lof_grouped* plan);
 LOF_API int32_t lof_grouped_num_configs(void);
 LOF_API lof_status lof_grouped_config_name(int32_t config, char* buf, int32_t buflen);
-/* Total work tiles and persistent grid size of a plan. */
+/* Work tiles of the persistent launch (device-filling GEMMs, launched on their own, are
+ * not counted) and its grid size. */
 LOF_API lof_status lof_grouped_info(const lof_grouped* plan, int64_t* tiles, int32_t* grid);
 /* Autotuner for the grouped launch: times every config on these items (outputs are
  * overwritten) and returns the best. */
@@ -312,7 +317,8 @@ LOF_API lof_status lof_grouped_autotune(const lof_group_item* items, int32_t cou
  * memory (tight serving loops, harnesses that forbid hipMalloc).  The work queue, the job
  * start table and the task records live in one fixed device workspace of the library
  * (about 170 KB per GPU, held until the process ends), so count <= 256 items
- * (LOF_ERR_UNSUPPORTED beyond).  The task table is copied from host memory into the
+ * (LOF_ERR_UNSUPPORTED beyond).  Device-filling GEMMs are launched on their own first, as
+ * in lof_grouped_run; the task table of the rest is copied from host memory into the
  * workspace on `stream` (a pageable copy, which HIP performs synchronously), then one
  * kernel runs it; outputs are bit-identical to lof_grouped_run on the same items.  Every
  * call on a GPU shares that GPU's workspace: calls that overlap on one GPU (other streams,

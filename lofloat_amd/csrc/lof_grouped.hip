@@ -250,6 +250,7 @@ struct lof_grouped {
     int64_t* d_starts = nullptr;
     unsigned long long* d_counters = nullptr;
     std::vector<GroupTask> tasks;   // host copy, in queue order
+    std::vector<GroupTask> direct;  // device-filling GEMMs, each run as its own launch
 };
 
 struct lof_graph {
@@ -323,11 +324,16 @@ int resolve_gconfig(int config) {
 }
 
 // Validates every item and lays the jobs out in queue order: GEMMs first, largest first,
-// then rounding jobs, largest first; jobs with no tiles are dropped.  work / tiles / order
-// are caller-provided scratch of `count` entries; tasks / starts receive *ntasks entries.
+// then rounding jobs, largest first; jobs with no tiles are dropped.  A GEMM with at least
+// `fill` tiles (the persistent grid's workgroups: it fills the device by itself) goes to
+// `direct` instead, to run as its own launch of the single-GEMM kernel, whose tile outruns
+// the combined kernel's (its registers are its own); the outputs are the same either way.
+// work / tiles / order are caller-provided scratch of `count` entries; tasks / starts
+// receive *ntasks entries, direct *ndirect.
 lof_status order_tasks(const lof_group_item* items, int count, const GroupedConfig& cfg,
-                       GroupTask* work, int64_t* tiles, int* order, GroupTask* tasks,
-                       int64_t* starts, int* ntasks, int64_t* total) {
+                       int64_t fill, GroupTask* work, int64_t* tiles, int* order,
+                       GroupTask* tasks, int64_t* starts, int* ntasks, int64_t* total,
+                       GroupTask* direct, int* ndirect) {
     for (int i = 0; i < count; ++i) {
         lof_status s = build_task(items[i], cfg, &work[i], &tiles[i]);
         if (s != LOF_OK) return s;
@@ -336,11 +342,15 @@ lof_status order_tasks(const lof_group_item* items, int count, const GroupedConf
     std::stable_sort(order, order + count, [&](int a, int b) {
         return task_cost(work[a], tiles[a]) > task_cost(work[b], tiles[b]);
     });
-    int nt = 0;
+    int nt = 0, nd = 0;
     int64_t tot = 0;
     for (int i = 0; i < count; ++i) {
         const int idx = order[i];
         if (tiles[idx] == 0) continue;
+        if (work[idx].kind == LOF_ITEM_GEMM && tiles[idx] >= fill) {
+            direct[nd++] = work[idx];
+            continue;
+        }
         starts[nt] = tot;
         tasks[nt] = work[idx];
         tot += tiles[idx];
@@ -350,11 +360,13 @@ lof_status order_tasks(const lof_group_item* items, int count, const GroupedConf
         return set_error(LOF_ERR_UNSUPPORTED, "lof_grouped: more than 2^30 tiles");
     *ntasks = nt;
     *total = tot;
+    *ndirect = nd;
     return LOF_OK;
 }
 
-// Persistent grid: every workgroup the device holds at once, at most one per tile.
-lof_status persistent_grid(const GroupedConfig& cfg, int device, int64_t total, int* grid) {
+// Workgroups of the config's kernel the device holds at once: the persistent grid is that
+// many, at most one per tile.
+lof_status resident_slots(const GroupedConfig& cfg, int device, int64_t* resident) {
     int cus = 0, per_cu = 0;
     lof_status s = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                                    device), "device attribute");
@@ -362,11 +374,29 @@ lof_status persistent_grid(const GroupedConfig& cfg, int device, int64_t total, 
         s = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                           &per_cu, reinterpret_cast<const void*>(cfg.kernel), 256, 0),
                       "occupancy");
-    if (s == LOF_OK) {
-        const int64_t resident = (int64_t)cus * std::max(per_cu, 1);
-        *grid = (int)std::max<int64_t>(1, std::min<int64_t>(resident, total));
-    }
+    if (s == LOF_OK) *resident = (int64_t)cus * std::max(per_cu, 1);
     return s;
+}
+
+int persistent_grid(int64_t resident, int64_t total) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(resident, total));
+}
+
+// One job through its own single-job entry point.
+lof_status launch_alone(const GroupTask& t, hipStream_t st) {
+    switch (t.kind) {
+    case LOF_ITEM_ROUND: return launch_round(t.r, false, t.mode, t.rsrc, st);
+    case LOF_ITEM_ROUND_MANT: return launch_round(t.r, true, t.mode, t.rsrc, st);
+    case LOF_ITEM_MX: return launch_mx(t.x, t.mode, t.rsrc, st);
+    default: {
+        lof_gemm_desc d{};
+        d.M = t.g.M; d.N = t.g.N; d.K = t.g.K;
+        d.trans_a = t.g.ta; d.trans_b = t.g.tb;
+        d.accum.kind = t.g.acc_kind;
+        d.accum.chunk = t.g.chunk;
+        return launch_gemm(t.g, resolve_gemm_config(d, -1), st);
+    }
+    }
 }
 
 // Host state of lof_grouped_run_noalloc: each device's workspace address and one host image
@@ -375,6 +405,7 @@ constexpr int kNoallocMaxDevices = 64;
 struct NoallocImage {
     alignas(256) unsigned char bytes[kNoallocBytes];
     GroupTask work[kNoallocMaxItems];
+    GroupTask direct[kNoallocMaxItems];
     int64_t tiles[kNoallocMaxItems];
     int order[kNoallocMaxItems];
 };
@@ -407,18 +438,25 @@ LOF_API lof_status lof_grouped_create(const lof_group_item* items, int32_t count
     auto* p = new lof_grouped();
     p->config = config;
     p->tasks.resize((size_t)count);
+    p->direct.resize((size_t)count);
     std::vector<int64_t> starts((size_t)count);
     std::vector<GroupTask> work((size_t)count);
     std::vector<int64_t> tiles((size_t)count);
     std::vector<int> order((size_t)count);
-    lof_status s = order_tasks(items, count, cfg, work.data(), tiles.data(), order.data(),
-                               p->tasks.data(), starts.data(), &p->ntasks, &p->total);
+    int64_t resident = 0;
+    int ndirect = 0;
+    lof_status s = check_hip(hipGetDevice(&p->device), "hipGetDevice");
+    if (s == LOF_OK) s = resident_slots(cfg, p->device, &resident);
+    if (s == LOF_OK)
+        s = order_tasks(items, count, cfg, resident, work.data(), tiles.data(), order.data(),
+                        p->tasks.data(), starts.data(), &p->ntasks, &p->total,
+                        p->direct.data(), &ndirect);
     if (s == LOF_OK) {
         p->tasks.resize((size_t)p->ntasks);
+        p->direct.resize((size_t)ndirect);
         starts.resize((size_t)p->ntasks);
-        s = check_hip(hipGetDevice(&p->device), "hipGetDevice");
+        p->grid = persistent_grid(resident, p->total);
     }
-    if (s == LOF_OK) s = persistent_grid(cfg, p->device, p->total, &p->grid);
     if (s == LOF_OK && p->ntasks > 0) {
         s = check_hip(hipMalloc(&p->d_tasks, sizeof(GroupTask) * p->tasks.size()), "hipMalloc");
         if (s == LOF_OK)
@@ -448,6 +486,10 @@ LOF_API lof_status lof_grouped_create(const lof_group_item* items, int32_t count
 
 LOF_API lof_status lof_grouped_run(lof_grouped* p, lof_stream_t stream) {
     if (!p) return set_error(LOF_ERR_INVALID_ARGUMENT, "lof_grouped_run: NULL plan");
+    for (const GroupTask& t : p->direct) {
+        const lof_status s = launch_alone(t, (hipStream_t)stream);
+        if (s != LOF_OK) return s;
+    }
     if (p->total == 0) return LOF_OK;
     const GroupedConfig& cfg = kGConfigs[p->config];
     hipLaunchKernelGGL(cfg.kernel, dim3((unsigned)p->grid), dim3(256), 0, (hipStream_t)stream,
@@ -458,23 +500,11 @@ LOF_API lof_status lof_grouped_run(lof_grouped* p, lof_stream_t stream) {
 LOF_API lof_status lof_grouped_run_separate(lof_grouped* p, lof_stream_t stream) {
     if (!p) return set_error(LOF_ERR_INVALID_ARGUMENT, "lof_grouped_run_separate: NULL plan");
     hipStream_t st = (hipStream_t)stream;
-    for (const GroupTask& t : p->tasks) {
-        lof_status s;
-        switch (t.kind) {
-        case LOF_ITEM_ROUND: s = launch_round(t.r, false, t.mode, t.rsrc, st); break;
-        case LOF_ITEM_ROUND_MANT: s = launch_round(t.r, true, t.mode, t.rsrc, st); break;
-        case LOF_ITEM_MX: s = launch_mx(t.x, t.mode, t.rsrc, st); break;
-        default: {
-            lof_gemm_desc d{};
-            d.M = t.g.M; d.N = t.g.N; d.K = t.g.K;
-            d.trans_a = t.g.ta; d.trans_b = t.g.tb;
-            d.accum.kind = t.g.acc_kind;
-            d.accum.chunk = t.g.chunk;
-            s = launch_gemm(t.g, resolve_gemm_config(d, -1), st);
+    for (const auto* list : {&p->direct, &p->tasks})
+        for (const GroupTask& t : *list) {
+            const lof_status s = launch_alone(t, st);
+            if (s != LOF_OK) return s;
         }
-        }
-        if (s != LOF_OK) return s;
-    }
     return LOF_OK;
 }
 
@@ -556,20 +586,25 @@ LOF_API lof_status lof_grouped_run_noalloc(const lof_group_item* items, int32_t 
         dptr = g_noalloc_ptr[device] = static_cast<unsigned char*>(ptr);
     }
     NoallocImage& im = g_noalloc_image;
-    int ntasks = 0;
-    int64_t total = 0;
-    s = order_tasks(items, count, cfg, im.work, im.tiles, im.order,
+    int ntasks = 0, ndirect = 0;
+    int64_t total = 0, resident = 0;
+    s = resident_slots(cfg, device, &resident);
+    if (s != LOF_OK) return s;
+    s = order_tasks(items, count, cfg, resident, im.work, im.tiles, im.order,
                     reinterpret_cast<GroupTask*>(im.bytes + kNoallocTasksOff),
-                    reinterpret_cast<int64_t*>(im.bytes + kNoallocStartsOff), &ntasks, &total);
+                    reinterpret_cast<int64_t*>(im.bytes + kNoallocStartsOff), &ntasks, &total,
+                    im.direct, &ndirect);
     if (s != LOF_OK) return s;
+    hipStream_t st = (hipStream_t)stream;
+    for (int i = 0; i < ndirect; ++i) {   // launches only: nothing is allocated
+        s = launch_alone(im.direct[i], st);
+        if (s != LOF_OK) return s;
+    }
     if (total == 0) return LOF_OK;
-    int grid = 0;
-    s = persistent_grid(cfg, device, total, &grid);
-    if (s != LOF_OK) return s;
+    const int grid = persistent_grid(resident, total);
     // counters start at zero (the kernel's last workgroup also leaves them at zero)
     std::memset(im.bytes, 0, kNoallocStartsOff);
     const size_t used = kNoallocTasksOff + (size_t)ntasks * sizeof(GroupTask);
-    hipStream_t st = (hipStream_t)stream;
     s = check_hip(hipMemcpyAsync(dptr, im.bytes, used, hipMemcpyHostToDevice, st),
                   "lof_grouped_run_noalloc upload");
     if (s != LOF_OK) return s;

@@ -128,6 +128,44 @@ def test_autotune_and_empty(dev):
     empty.run()
 
 
+def test_device_filling_gemm_runs_alone(dev):
+    """A GEMM with at least as many tiles as the persistent grid has workgroups runs as its
+    own launch of the single-GEMM kernel, ahead of the persistent launch: the plan's queue
+    then holds only the other jobs' tiles, and every output stays bit-identical to the
+    per-job launches (the small jobs also to the oracle), through lof_grouped_run, a graph
+    replay and lof_grouped_run_noalloc alike."""
+    import lofloat_amd as lof
+    jobs, refs = _make_jobs(lof, dev, seed=21, n_round=10, n_gemm=2)
+    g = np.random.default_rng(21)
+    e4m3 = (8, 3, 7, 1, 0, 0)
+    A = oracle.virtual_round((g.standard_normal((4096, 64)) / 64).astype(np.float32),
+                             e4m3).reshape(4096, 64)
+    B = oracle.virtual_round((g.standard_normal((64, 4096)) / 64).astype(np.float32),
+                             e4m3).reshape(64, 4096)
+    big = lof.GemmJob(torch.from_numpy(A).to(dev), torch.from_numpy(B).to(dev),
+                      lof.AccumSpec(fmt=_fmt((16, 10, 15, 1, 0, 0)), chunk=32))
+    small_tiles = lof.GroupedPlan(jobs).info()[0]
+    plan = lof.GroupedPlan(jobs + [big])
+    tiles, grid = plan.info()
+    assert tiles == small_tiles, (tiles, small_tiles)   # the 4096^2 GEMM left the queue
+    assert 4096 * 4096 // (128 * 128) >= grid            # it fills the grid in any config
+    plan.run_separate()
+    torch.cuda.synchronize()
+    want = big.out.cpu().numpy().copy()
+    _check(jobs, refs, "separate")
+    runs = {"grouped": plan.run, "noalloc": lambda: lof.grouped.run_noalloc(jobs + [big])}
+    graph = plan.capture_graph(separate=False)
+    runs["graph"] = graph.replay
+    for name, run in runs.items():
+        for j in jobs + [big]:
+            j.out.fill_(np.nan)
+        torch.cuda.synchronize()
+        run()
+        torch.cuda.synchronize()
+        _check(jobs, refs, name)
+        assert_bits(big.out.cpu().numpy().ravel(), want.ravel(), f"{name} big GEMM")
+
+
 def test_gemm_autotune(dev):
     import lofloat_amd as lof
     A = torch.randn(256, 192, device=dev)
