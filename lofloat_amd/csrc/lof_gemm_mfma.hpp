@@ -609,14 +609,17 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // and per float4 a wave-uniform step.  (Assigned through references in two branches, the
     // coordinate pair was kept in private memory, and its scratch reads -- whose waits also
     // wait for the in-flight prefetch -- sat in every slice's store.)
-    const int a_mb = g.ta ? 4 * (tid % (BM / 4)) : tid / 8;
-    const int a_kb = g.ta ? tid / (BM / 4) : 4 * (tid % 8);
-    const int b_nb = g.tb ? tid / 8 : 4 * (tid % (BN / 4));
-    const int b_kb = g.tb ? 4 * (tid % 8) : tid / (BN / 4);
-    auto a_coord = [&](int it, int j) -> int2 {   // (m, k)
+    // t: tid, recomputed per slice from an opaque copy (otid) so that the compiler does not
+    // hold the per-float4 coordinates and addresses across the loop in registers.
+    auto otid = [&]() { int t = tid; asm volatile("" : "+v"(t)); return t; };
+    auto a_coord = [&](int it, int j, int t) -> int2 {   // (m, k)
+        const int a_mb = g.ta ? 4 * (t % (BM / 4)) : t / 8;
+        const int a_kb = g.ta ? t / (BM / 4) : 4 * (t % 8);
         return make_int2(a_mb + (g.ta ? j : 32 * it), a_kb + (g.ta ? it * (1024 / BM) : j));
     };
-    auto b_coord = [&](int it, int j) -> int2 {   // (n, k)
+    auto b_coord = [&](int it, int j, int t) -> int2 {   // (n, k)
+        const int b_nb = g.tb ? t / 8 : 4 * (t % (BN / 4));
+        const int b_kb = g.tb ? 4 * (t % 8) : t / (BN / 4);
         return make_int2(b_nb + (g.tb ? 32 * it : j), b_kb + (g.tb ? j : it * (1024 / BN)));
     };
     // The edge loads' element coordinates, from a copy of tid the compiler cannot see
@@ -655,10 +658,11 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // B(k, n) for the slice rows k = k0 + f / (BN / 4): the tap (c, r, s) of k, then one
     // element per column j, zero where the tap falls in the padding or k >= K
     auto gather_b = [&](int k0) {
+        const int t = otid();
         const ConvGeom& cv = g.cv;
 #pragma unroll
         for (int it = 0; it < NB4; ++it) {
-            const int2 nk = b_coord(it, 0);
+            const int2 nk = b_coord(it, 0, t);
             const int n = nk.x, k = nk.y;
             const int gk = k0 + k;
             const uint32_t d1 = fast_div((uint32_t)gk, cv.div_d1);
@@ -683,11 +687,12 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         }
     };
     auto load_slice = [&](int k0) {
+        const int t = otid();
         const bool whole = g.vec && interior_mn && k0 + BK <= g.K;
         if (whole) {   // whole slice inside: float4 loads
 #pragma unroll
             for (int it = 0; it < NA4; ++it) {
-                const int2 mk = a_coord(it, 0);
+                const int2 mk = a_coord(it, 0, t);
                 const int m = mk.x, k = mk.y;
                 const float* p = g.ta ? g.A + (int64_t)(k0 + k) * g.lda + (m0 + m)
                                       : g.A + (int64_t)(m0 + m) * g.lda + (k0 + k);
@@ -714,7 +719,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         } else if (whole) {
 #pragma unroll
             for (int it = 0; it < NB4; ++it) {
-                const int2 nk = b_coord(it, 0);
+                const int2 nk = b_coord(it, 0, t);
                 const int n = nk.x, k = nk.y;
                 const float* p = g.tb ? g.B + (int64_t)(n0 + n) * g.ldb + (k0 + k)
                                       : g.B + (int64_t)(k0 + k) * g.ldb + (n0 + n);
@@ -739,13 +744,14 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     };
     // registers -> LDS buffer `buf`, with the slice's exactness statistics
     auto store_slice = [&](int buf) {
+        const int t = otid();
         float* As = smem + buf * (T::BUF_FLOATS + T::STAT_WORDS);
         float* Bs = As + BK * LDA_S;
         uint32_t* stat = (uint32_t*)(Bs + BK * LDB_S);
         SliceStat<kKey> sa, sb;
 #pragma unroll
         for (int it = 0; it < NA4; ++it) {
-            const int2 mk = a_coord(it, 0);
+            const int2 mk = a_coord(it, 0, t);
             const int m = mk.x, k = mk.y;
             if (g.ta) {
                 *(float4*)&As[k * LDA_S + m] = va[it];
@@ -759,7 +765,7 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
         }
 #pragma unroll
         for (int it = 0; it < NB4; ++it) {
-            const int2 nk = b_coord(it, 0);
+            const int2 nk = b_coord(it, 0, t);
             const int n = nk.x, k = nk.y;
             if (g.tb) {
                 Bs[(k + 0) * LDB_S + n] = vb[it].x;
