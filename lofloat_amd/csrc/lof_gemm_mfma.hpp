@@ -558,15 +558,14 @@ __device__ __forceinline__ void gemm_tile_mfma(const GemmArgs& g, int tile_m, in
     // FP32's), so the other kernels leave out their chunk-sum leaves -- except at chunk 32,
     // whose kernel measured 129 registers of spills without them (a different schedule)
     constexpr bool kF16Leaves = (SUBN || CHUNK >= 32) && !DRC;
-    // the general leaf's arguments by value in the 64-row tiles: their tile loop then holds
-    // GemmArgs in registers instead of reading it back from scratch (whose waits also wait
-    // for the in-flight prefetch; measured 30% faster at chunk 32, 10-25% at chunk 1).  The
-    // other tiles keep the reference, which costs them fewer registers (measured: the
-    // 128-row tiles lose occupancy by the value, the 64x128 tile at chunks 2 and 4 loses up
-    // to 20%, the DRC tile of chunk 8 5% (chunk 16 keeps the reference with it), while the
-    // chunk-32 DRC tile gains 12%).
-    constexpr bool kLeafByVal = WMB == 1 && !GMEM && (!DRC || CHUNK == 32) &&
-                                (WNB == 1 || CHUNK == 1 || CHUNK >= 8);
+    // the general leaf's arguments by value: the tile loop then holds GemmArgs in registers
+    // instead of reading it back from scratch (whose waits also wait for the in-flight
+    // prefetch; measured 30% faster on the 64x128 tile at chunk 32, 10-25% at chunk 1, 24%
+    // on the 128-row tiles at chunk 32).  Tiles where the values cost more registers than
+    // the copy keep the reference (measured: by value the 128-row tiles at chunk 1 lose a
+    // wave per SIMD, the 64x128 tile at chunks 2 and 4 runs 3-20% slower).
+    constexpr bool kLeafByVal = !GMEM && !(WMB == 2 && CHUNK == 1) &&
+                                !(WMB == 1 && WNB == 2 && (CHUNK == 2 || CHUNK == 4));
 
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
