@@ -10,7 +10,8 @@ with e2m1 and with SR-A random words; the emulated-accumulation GEMM at M=N=K=40
 (TFLOP/s = 2MNK / t) for 10-bit mantissa and 5e10m accumulators at chunks 1/8/32; one
 LoF_Linear forward+backward step (X 8192x4096, W 4096x4096); and the grouped launch of 64
 uneven rounding jobs + 12 GEMMs (e4m3 operands) against per-job launches, both captured in
-HIP graphs; the bit-pack codec encode / decode of 2^28 values to e4m3fn and to an 11-bit
+HIP graphs, and the same on a launch-bound batch of 240 small rounding / MX jobs + 16 small
+GEMMs; the bit-pack codec encode / decode of 2^28 values to e4m3fn and to an 11-bit
 P3109 format; a 3x3 Conv2d layer as an implicit GEMM against unfold + GEMM.
 
 The CPU baseline is the C oracle (a restatement of the reference's scalar semantics) run
@@ -502,9 +503,43 @@ def c5_jobs(lof, torch, np, dev):
     return jobs, gemm_flop
 
 
+def small_jobs(lof, torch, np, dev):
+    """A launch-bound batch: 240 rounding / MX jobs of 2^10 .. 2^16 elements, each with its
+    own format (exponent 2-8, mantissa 1-10 bits) and mode, and 16 emulated GEMMs of 64 .. 256
+    per side (operands N(0, 1/64) pre-rounded to e4m3, 5e10m accumulator at chunk 32).
+    Seeded; returns the jobs."""
+    e4m3 = lof.make_format(8, 3, 7)
+    fp16 = lof.make_format(16, 10, 15)
+    rng = np.random.default_rng(9)
+    gen = torch.Generator(device=dev).manual_seed(99)
+    jobs = []
+    modes = [0, 12, 4, 1, 3]
+    for i in range(240):
+        nn = int(np.exp2(rng.uniform(10, 16)))
+        e, m = int(rng.integers(2, 9)), int(rng.integers(1, 11))
+        mode = modes[i % 5]
+        xi = torch.randn(nn, device=dev, generator=gen)
+        bt = (torch.randint(-2**31, 2**31 - 1, (nn,), dtype=torch.int32, device=dev,
+                            generator=gen) if mode == 3 else None)
+        L = min(8, 23 - m) if mode == 3 else 0
+        if i % 4 == 3 and nn >= 64:
+            jobs.append(lof.MXJob(xi, 32, e4m3, lof.create_e8m0_params(), mode, 0, L,
+                                  rand_bits=bt))
+        else:
+            jobs.append(lof.RoundJob(xi, lof.make_format(1 + e + m, m, (1 << (e - 1)) - 1),
+                                     mode, L, rand_bits=bt))
+    for j in range(16):
+        Mg, Ng, Kg = (int(rng.choice([64, 128, 192, 256])) for _ in range(3))
+        A = lof.virtual_round(torch.randn(Mg, Kg, device=dev, generator=gen) / 64, e4m3)
+        B = lof.virtual_round(torch.randn(Kg, Ng, device=dev, generator=gen) / 64, e4m3)
+        jobs.append(lof.GemmJob(A, B, lof.AccumSpec(fmt=fp16, chunk=32)))
+    return jobs
+
+
 def grouped_vs_separate(lof, torch, np, dev, out):
     """The batched workload (c5_jobs): one persistent launch vs per-job launches, both
-    captured in HIP graphs."""
+    captured in HIP graphs; then the same comparison on a launch-bound batch of many small
+    jobs (small_jobs)."""
     jobs, gemm_flop = c5_jobs(lof, torch, np, dev)
     cfg, tune_ms = lof.grouped.autotune(jobs, iters=1)
     plan = lof.GroupedPlan(jobs)
@@ -520,6 +555,20 @@ def grouped_vs_separate(lof, torch, np, dev, out):
                 "grouped_round_elems": int(sum(j.input.numel() for j in jobs[:64])),
                 "grouped_gemm_TFLOP": round(gemm_flop / 1e12, 3)})
     log(f"grouped {ms_g:.1f} ms vs separate {ms_s:.1f} ms, bit-exact {same}")
+    jobs = small_jobs(lof, torch, np, dev)
+    plan = lof.GroupedPlan(jobs)
+    gg = plan.capture_graph(separate=False)
+    gs = plan.capture_graph(separate=True)
+    ms_g = timed(lambda i: gg.replay(), 20)
+    outs_g = [j.out.clone() for j in jobs]
+    ms_s = timed(lambda i: gs.replay(), 20)
+    same = all(torch.equal(a, j.out) for a, j in zip(outs_g, jobs))
+    out.update({"grouped_small_jobs_ms": round(ms_g, 3), "separate_small_jobs_ms": round(ms_s, 3),
+                "grouped_small_jobs_speedup": round(ms_s / ms_g, 2),
+                "grouped_small_jobs_bitexact_vs_separate": bool(same),
+                "grouped_small_jobs_count": len(jobs)})
+    log(f"grouped small jobs ({len(jobs)}) {ms_g:.3f} ms vs separate {ms_s:.3f} ms, "
+        f"bit-exact {same}")
 
 
 if __name__ == "__main__":
