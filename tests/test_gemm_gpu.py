@@ -149,16 +149,21 @@ def test_mfma_forced(dev, cfg):
 
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 def test_layouts(dev, ta, tb):
+    """Transposed operands on every tile kind: a ragged shape (edge loads everywhere) and one
+    whose tiles are all interior (the float4 slice loads, whose per-thread coordinates differ
+    by layout and by tile shape), at chunks 1, 8 and 32."""
     import lofloat_amd as lof
-    A, B = _mats(96, 80, 112, seed=5)
-    for chunk in (1, 8):
-        acc, oacc = _accums(lof, 10, chunk)
-        ref = _ref(A, B, oacc)
-        for cfg in (0, 3, 5, 7):
-            if not lof.gemm_config_valid(cfg, chunk):
-                continue
-            y = _run(lof, A, B, acc, dev, ta, tb, config=cfg)
-            assert_bits(y, ref, f"ta={ta} tb={tb} chunk {chunk} cfg {cfg}")
+    for M, N, K, kind in ((96, 80, 112, "normal"), (256, 256, 128, "normal"),
+                          (256, 256, 128, "e4m3_small")):
+        A, B = _mats(M, N, K, seed=5, kind=kind)
+        for chunk in (1, 8, 32):
+            acc, oacc = _accums(lof, 10, chunk)
+            ref = _ref(A, B, oacc)
+            for cfg in (0, 3, 5, 6, 7, 8):
+                if not lof.gemm_config_valid(cfg, chunk):
+                    continue
+                y = _run(lof, A, B, acc, dev, ta, tb, config=cfg)
+                assert_bits(y, ref, f"{M}x{N}x{K} {kind} ta={ta} tb={tb} chunk {chunk} cfg {cfg}")
 
 
 @pytest.mark.parametrize("m", [0, 1, 3, 7, 10, 15, 19, 20, 21, 22, 23])
